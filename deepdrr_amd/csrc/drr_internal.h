@@ -1,0 +1,114 @@
+// drr_internal.h -- shared definitions of the HIP library (not installed).
+//
+// Volume layout in HBM ("cell-expanded"): for a volume of nx*ny*nz voxels the
+// library stores one record per trilinear cell (i, j, k), i in [-1, nx-1]
+// (same for j, k), i.e. (nx+1)(ny+1)(nz+1) cells, i fastest:
+//   dens[cell]  : 8 float32 corner densities, corner d = dx + 2 dy + 4 dz
+//                 holding D[clamp(i+dx)][clamp(j+dy)][clamp(k+dz)]   (32 B)
+//   labels[cell]: 8 uint8 corner labels, same corner order           (8 B)
+// A ray sample with base = floor(p) reads cell clamp(base, -1, n-1): two
+// aligned 16-byte loads and one 8-byte load replace the 16 scattered texel
+// fetches of the reference's texture path (project_kernel.cu:421-425, 542).
+// Clamping the cell index reproduces clamp-to-edge addressing exactly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/deepdrr_amd.h"
+
+struct drr_volume_s {
+    drr_ctx *ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n_cells = 0;
+    float4 *dens = nullptr;     // 2 float4 per cell
+    uint2 *labels = nullptr;    // 8 bytes per cell
+};
+
+struct drr_ctx_s {
+    int device = 0;
+    drr_scene_t scene{0.1f, 0, 0, 0};
+    bool have_scene = false;
+    int n_bins = 0;
+    float *d_energies = nullptr;  // [B]
+    float *d_pdf = nullptr;       // [B]
+    float *d_mu = nullptr;        // [B * M]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+// ---- error plumbing (defined in runtime.hip) ------------------------------
+int drr_set_error(int code, const std::string &msg);
+int drr_hip_error(hipError_t e, const char *what);
+
+#define DRR_HIP_CHECK(expr)                                   \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return drr_hip_error(_e, #expr); \
+    } while (0)
+
+// ---- kernel parameter blocks ------------------------------------------------
+
+// One volume as the march sees it.
+struct VolDev {
+    const float4 *dens;
+    const uint2 *labels;
+    int nx, ny, nz;
+};
+
+// Everything the per-ray march needs for one view of up to V volumes.
+struct ViewDev {
+    float W[9];
+    float max_ray_length;
+    int width, height;
+    float A[DRR_MAX_VOLUMES][12];
+    float s[DRR_MAX_VOLUMES][3];
+    float lo[DRR_MAX_VOLUMES][3];
+    float hi[DRR_MAX_VOLUMES][3];
+    int prio[DRR_MAX_VOLUMES];
+    int en[DRR_MAX_VOLUMES];
+    VolDev vol[DRR_MAX_VOLUMES];
+};
+
+struct SpectrumDev {
+    const float *energies;
+    const float *pdf;
+    const float *mu;
+    int n_bins;
+};
+
+inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, ViewDev &d) {
+    for (int i = 0; i < 9; i++) d.W[i] = v.world_from_index[i];
+    d.max_ray_length = v.max_ray_length;
+    d.width = v.width;
+    d.height = v.height;
+    for (int k = 0; k < DRR_MAX_VOLUMES; k++) {
+        const drr_view_volume_t &vv = v.vol[k];
+        for (int i = 0; i < 12; i++) d.A[k][i] = vv.ijk_from_world[i];
+        for (int i = 0; i < 3; i++) {
+            d.s[k][i] = vv.source_ijk[i];
+            d.lo[k][i] = vv.bounds_lo[i];
+            d.hi[k][i] = vv.bounds_hi[i];
+        }
+        d.prio[k] = vv.priority;
+        d.en[k] = vv.enabled;
+        if (k < v.n_volumes && vols && vols[k]) {
+            d.vol[k].dens = vols[k]->dens;
+            d.vol[k].labels = vols[k]->labels;
+            d.vol[k].nx = vols[k]->nx;
+            d.vol[k].ny = vols[k]->ny;
+            d.vol[k].nz = vols[k]->nz;
+        } else {
+            d.vol[k] = VolDev{nullptr, nullptr, 1, 1, 1};
+        }
+    }
+}
+
+// Validation shared by the entry points.
+int drr_check_view(const drr_ctx *ctx, const drr_volume *const *vols, const drr_view_t *view);
+
+// Launchers implemented in forward.hip (used by batch/adjoint code).
+int drr_launch_expand(const float *density, const uint8_t *labels, int nx, int ny, int nz,
+                      float4 *dens, uint2 *lab, hipStream_t st);

@@ -1,0 +1,484 @@
+// march.h -- per-ray DRR march on CDNA4, shared by the plain, batched and
+// adjoint kernels.
+//
+// Semantics are those of deepdrr's projectKernel
+// (deepdrr/projector/project_kernel.cu:195-646): same FP32 operation order,
+// compiled with -ffp-contract=off, IEEE division and sqrt.  Differences are
+// in *how* the work is done, never in the values:
+//   * the 8 label corners and 8 density corners of a trilinear cell are read
+//     once per cell change from the cell-expanded layout (drr_internal.h) and
+//     kept in registers while the ray stays in the cell; the reference keeps
+//     the labels (project_kernel.cu:394-431) and re-reads density texels at
+//     every step (542);
+//   * for a single volume the steps before the volume are skipped with the
+//     same sequential FP32 `alpha += step` (552) but without sampling; the
+//     label cache is re-established inside a window of at least one voxel
+//     before entry, which reproduces the cache state of the reference (whose
+//     fetch point inside a cell decides the labels when p + 1 rounds up);
+//   * a cell whose 8 labels agree sums the weights once instead of per
+//     material; adding 0.0f to a non-negative accumulator is exact, so the
+//     result is bit-identical to the per-material sums (451-455, 543-547).
+#pragma once
+
+#include "drr_internal.h"
+
+#define DRR_AIR_DENSITY 0.1129  // double, as project_kernel.cu:6
+
+namespace drr {
+
+__device__ __forceinline__ float add_air(float a) {
+    // area_density[AIR] += AIR_DENSITY evaluates in double (project_kernel.cu:526)
+    return (float)((double)a + DRR_AIR_DENSITY);
+}
+
+__device__ __forceinline__ void ray_dir(const float *W, int udx, int vdx, float &rx, float &ry,
+                                        float &rz, float &L) {
+    // (float)udx + 0.5 (a double literal) is exact for udx < 2^23.
+    const float u = (float)udx + 0.5f;
+    const float v = (float)vdx + 0.5f;
+    rx = u * W[0] + v * W[1] + W[2];
+    ry = u * W[3] + v * W[4] + W[5];
+    rz = u * W[6] + v * W[7] + W[8];
+    L = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float inv = 1.0f / L;
+    rx *= inv;
+    ry *= inv;
+    rz *= inv;
+}
+
+// Slab clip of one volume (project_kernel.cu:274-315).  Returns do_trace.
+__device__ __forceinline__ bool slab(const float *A, const float *s, const float *lo,
+                                     const float *hi, float mrl, float rx, float ry, float rz,
+                                     float ri[3], float &mn, float &mx) {
+    ri[0] = A[0] * rx + A[1] * ry + A[2] * rz + A[3] * 0.0f;
+    ri[1] = A[4] * rx + A[5] * ry + A[6] * rz + A[7] * 0.0f;
+    ri[2] = A[8] * rx + A[9] * ry + A[10] * rz + A[11] * 0.0f;
+    mn = 0.0f;
+    mx = mrl > 0 ? mrl : INFINITY;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (0.0f != ri[a]) {
+            const float reci = 1.0f / ri[a];
+            const float a0 = (lo[a] - s[a]) * reci;
+            const float a1 = (hi[a] - s[a]) * reci;
+            mn = fmaxf(mn, fminf(a0, a1));
+            mx = fminf(mx, fmaxf(a0, a1));
+        } else if (lo[a] > s[a] || s[a] > hi[a]) {
+            return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ int clamp_cell(float b, int n) {
+    // cell index in [-1, n-1]; clamp in float so huge/NaN values stay defined
+    const float c = fminf(fmaxf(b, -1.0f), (float)(n - 1));
+    return (int)c;
+}
+
+__device__ __forceinline__ int64_t cell_index(const VolDev &v, int cx, int cy, int cz) {
+    return ((int64_t)(cz + 1) * (v.ny + 1) + (cy + 1)) * (int64_t)(v.nx + 1) + (cx + 1);
+}
+
+__device__ __forceinline__ int clamp_voxel(float f, int n) {
+    const float c = fminf(fmaxf(f, 0.0f), (float)(n - 1));
+    return (int)c;
+}
+
+// Label of voxel (clamp(floor(x)), ...) = corner 0 of that voxel's cell.
+__device__ __forceinline__ uint32_t label_point(const VolDev &v, float x, float y, float z) {
+    const int i = clamp_voxel(floorf(x), v.nx);
+    const int j = clamp_voxel(floorf(y), v.ny);
+    const int k = clamp_voxel(floorf(z), v.nz);
+    return v.labels[cell_index(v, i, j, k)].x & 0xffu;
+}
+
+// Labels of the 8 corners for sample p with base b: L[clamp(floor(p + d))]
+// (point sampling at p + d, project_kernel.cu:421-425).  Packed 4 per word.
+__device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py, float pz,
+                                             float bx, float by, float bz, int64_t cell,
+                                             uint32_t &l0, uint32_t &l1) {
+    const bool regular = (floorf(px + 1.0f) == bx + 1.0f) && (floorf(py + 1.0f) == by + 1.0f) &&
+                         (floorf(pz + 1.0f) == bz + 1.0f);
+    if (regular) {
+        const uint2 lab = v.labels[cell];
+        l0 = lab.x;
+        l1 = lab.y;
+    } else {
+        uint32_t b[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const float dx = (float)(d & 1), dy = (float)((d >> 1) & 1), dz = (float)(d >> 2);
+            b[d] = label_point(v, px + dx, py + dy, pz + dz);
+        }
+        l0 = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+        l1 = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
+    }
+}
+
+__device__ __forceinline__ void weights8(float fx, float fy, float fz, float w[8]) {
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    w[0] = gx * gy * gz;
+    w[1] = fx * gy * gz;
+    w[2] = gx * fy * gz;
+    w[3] = fx * fy * gz;
+    w[4] = gx * gy * fz;
+    w[5] = fx * gy * fz;
+    w[6] = gx * fy * fz;
+    w[7] = fx * fy * fz;
+}
+
+__device__ __forceinline__ uint32_t corner_label(uint32_t l0, uint32_t l1, int d) {
+    return ((d < 4 ? l0 : l1) >> (8 * (d & 3))) & 0xffu;
+}
+
+// seg[m] = sum over corners (dz, dy, dx order) of w[d] where label == m.
+template <int M>
+__device__ __forceinline__ void material_fractions(uint32_t l0, uint32_t l1, const float w[8],
+                                                   float seg[M]) {
+    const uint32_t m0 = l0 & 0xffu;
+    const bool uniform = (l0 == l1) && (l0 == m0 * 0x01010101u);
+    if (uniform) {
+        const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
+#pragma unroll
+        for (int m = 0; m < M; m++) seg[m] = ((uint32_t)m == m0) ? ssum : 0.0f;
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 8; d++) s += (corner_label(l0, l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+            seg[m] = s;
+        }
+    }
+}
+
+__device__ __forceinline__ float trilerp(const float4 &d0, const float4 &d1, const float w[8]) {
+    float rho = 0.0f;
+    rho += w[0] * d0.x;
+    rho += w[1] * d0.y;
+    rho += w[2] * d0.z;
+    rho += w[3] * d0.w;
+    rho += w[4] * d1.x;
+    rho += w[5] * d1.y;
+    rho += w[6] * d1.z;
+    rho += w[7] * d1.w;
+    return rho;
+}
+
+template <int M>
+__device__ __forceinline__ float get_m(const float (&a)[M], int idx) {
+    float r = 0.0f;
+#pragma unroll
+    for (int m = 0; m < M; m++) r = (m == idx) ? a[m] : r;
+    return r;
+}
+
+template <int M>
+__device__ __forceinline__ void set_m(float (&a)[M], int idx, float val) {
+#pragma unroll
+    for (int m = 0; m < M; m++) a[m] = (m == idx) ? val : a[m];
+}
+
+// Per-ray state of the single-volume march, filled by march_setup_v1.
+struct RayV1 {
+    float rx, ry, rz, L;
+    float ri[3];
+    float mnv, mxv;
+    float minA, maxA;
+    int N;
+    bool trace;
+};
+
+__device__ __forceinline__ void march_setup_v1(const ViewDev &vd, int udx, int vdx, float step,
+                                               RayV1 &r) {
+    ray_dir(vd.W, udx, vdx, r.rx, r.ry, r.rz, r.L);
+    r.minA = r.L;
+    r.maxA = 0.0f;
+    r.trace = false;
+    r.ri[0] = r.ri[1] = r.ri[2] = 0.0f;
+    r.mnv = 0.0f;
+    r.mxv = 0.0f;
+    if (vd.en[0]) {
+        r.trace = slab(vd.A[0], vd.s[0], vd.lo[0], vd.hi[0], vd.max_ray_length, r.rx, r.ry, r.rz,
+                       r.ri, r.mnv, r.mxv);
+        if (r.trace) {
+            r.minA = fminf(r.minA, r.mnv);
+            r.maxA = fmaxf(r.maxA, r.mxv);
+        }
+    }
+    r.N = (int)ceilf((r.maxA - r.minA) / step);
+}
+
+// Single-volume march (project_kernel.cu:334-584 with NUM_VOLUMES == 1).
+// On return ad[] holds the final area densities in g/cm^2.
+template <int M, bool ATTEN>
+__device__ __forceinline__ void march_v1(const ViewDev &vd, const VolDev &vol, int udx, int vdx,
+                                         float step, int air, float (&ad)[M]) {
+    RayV1 r;
+    march_setup_v1(vd, udx, vdx, step, r);
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] = 0.0f;
+    float acc = 0.0f;  // air accumulator while no volume is sampled
+    if (ATTEN) acc = (float)((double)acc + (double)(r.minA / step) * DRR_AIR_DENSITY);
+
+    const int N = r.N;
+    float alpha = r.minA;
+    int t = 0;
+    if (r.trace) {
+        const float sx = vd.s[0][0], sy = vd.s[0][1], sz = vd.s[0][2];
+        const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
+        // Phase 1: steps well before the volume: only the alpha sequence.
+        const float rdom = fmaxf(fabsf(rix), fmaxf(fabsf(riy), fabsf(riz)));
+        const float lim = r.mnv - (1.0f / rdom + 2.0f * step);
+        if (ATTEN) {
+            while (t < N && alpha < lim) {
+                acc = add_air(acc);
+                alpha += step;
+                ++t;
+            }
+        } else {
+            while (t < N && alpha < lim) {
+                alpha += step;
+                ++t;
+            }
+        }
+        // Phase 2: approach window, label cache tracked as the reference does.
+        float pbx = 0.0f, pby = 0.0f, pbz = 0.0f;
+        bool have = false;
+        uint32_t l0 = 0, l1 = 0;
+        float4 d0 = make_float4(0, 0, 0, 0), d1 = d0;
+        while (t < N && alpha < r.mnv) {
+            const float px = sx + alpha * rix - 1.0f;
+            const float py = sy + alpha * riy - 1.0f;
+            const float pz = sz + alpha * riz - 1.0f;
+            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+            if (!have || bx != pbx || by != pby || bz != pbz) {
+                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                                                clamp_cell(bz, vol.nz));
+                fetch_labels(vol, px, py, pz, bx, by, bz, cell, l0, l1);
+                d0 = vol.dens[2 * cell];
+                d1 = vol.dens[2 * cell + 1];
+                pbx = bx;
+                pby = by;
+                pbz = bz;
+                have = true;
+            }
+            if (ATTEN) acc = add_air(acc);
+            alpha += step;
+            ++t;
+        }
+        if (ATTEN) set_m<M>(ad, air, acc);
+        // Phase 3: inside [minAlpha_vol, maxAlpha_vol]: sample and accumulate.
+        while (t < N && alpha <= r.mxv) {
+            const float px = sx + alpha * rix - 1.0f;
+            const float py = sy + alpha * riy - 1.0f;
+            const float pz = sz + alpha * riz - 1.0f;
+            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+            if (!have || bx != pbx || by != pby || bz != pbz) {
+                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                                                clamp_cell(bz, vol.nz));
+                fetch_labels(vol, px, py, pz, bx, by, bz, cell, l0, l1);
+                d0 = vol.dens[2 * cell];
+                d1 = vol.dens[2 * cell + 1];
+                pbx = bx;
+                pby = by;
+                pbz = bz;
+                have = true;
+            }
+            float w[8];
+            weights8(px - bx, py - by, pz - bz, w);
+            float seg[M];
+            material_fractions<M>(l0, l1, w, seg);
+            const float rho = trilerp(d0, d1, w);
+            const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
+            const float x = wt * rho;
+#pragma unroll
+            for (int m = 0; m < M; m++) ad[m] += x * seg[m];
+            alpha += step;
+            ++t;
+        }
+        if (ATTEN) acc = get_m<M>(ad, air);
+    }
+    // Phase 4: the rest of the steps sample nothing.
+    if (ATTEN) {
+        for (; t < N; ++t) acc = add_air(acc);
+        acc = (float)((double)acc + (double)((r.L - r.maxA) / step) * DRR_AIR_DENSITY);
+        set_m<M>(ad, air, acc);
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] *= step;
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] /= 10.0f;
+}
+
+// General march for 1..V volumes with priorities and the label cache shared
+// across volumes (project_kernel.cu:394-553), literal step by step.
+template <int V, int M>
+__device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx, float step,
+                                            int atten, int air, float (&ad)[M]) {
+    float rx, ry, rz, L;
+    ray_dir(vd.W, udx, vdx, rx, ry, rz, L);
+    float minA = L, maxA = 0.0f;
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] = 0.0f;
+    float ri[V][3], mnv[V], mxv[V];
+    bool trace[V];
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+        ri[i][0] = ri[i][1] = ri[i][2] = 0.0f;
+        mnv[i] = mxv[i] = 0.0f;
+        trace[i] = false;
+        if (!vd.en[i]) continue;
+        trace[i] = slab(vd.A[i], vd.s[i], vd.lo[i], vd.hi[i], vd.max_ray_length, rx, ry, rz, ri[i],
+                        mnv[i], mxv[i]);
+        if (trace[i]) {
+            minA = fminf(minA, mnv[i]);
+            maxA = fmaxf(maxA, mxv[i]);
+        }
+    }
+    const int N = (int)ceilf((maxA - minA) / step);
+    float acc_air = 0.0f;
+    if (atten) acc_air = (float)((double)acc_air + (double)(minA / step) * DRR_AIR_DENSITY);
+    if (atten) set_m<M>(ad, air, acc_air);
+
+    float pbx = 0.0f, pby = 0.0f, pbz = 0.0f;
+    bool have = false;
+    uint32_t l0 = 0, l1 = 0;
+    float alpha = minA;
+    for (int t = 0; t < N; t++) {
+        float p[V][3];
+        float seg[V][M];
+#pragma unroll
+        for (int i = 0; i < V; i++) {
+#pragma unroll
+            for (int m = 0; m < M; m++) seg[i][m] = 0.0f;
+            p[i][0] = p[i][1] = p[i][2] = 0.0f;
+            if (!vd.en[i]) continue;
+            const VolDev &vol = vd.vol[i];
+            p[i][0] = vd.s[i][0] + alpha * ri[i][0] - 1.0f;
+            p[i][1] = vd.s[i][1] + alpha * ri[i][1] - 1.0f;
+            p[i][2] = vd.s[i][2] + alpha * ri[i][2] - 1.0f;
+            const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
+            if (!have || bx != pbx || by != pby || bz != pbz) {
+                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                                                clamp_cell(bz, vol.nz));
+                fetch_labels(vol, p[i][0], p[i][1], p[i][2], bx, by, bz, cell, l0, l1);
+                pbx = bx;
+                pby = by;
+                pbz = bz;
+                have = true;
+            }
+            float w[8];
+            weights8(p[i][0] - bx, p[i][1] - by, p[i][2] - bz, w);
+            material_fractions<M>(l0, l1, w, seg[i]);
+        }
+        int cp = V, n_at = 0;
+#pragma unroll
+        for (int i = 0; i < V; i++) {
+            if (!trace[i] || !vd.en[i]) continue;
+            if ((alpha < mnv[i]) || (alpha > mxv[i])) continue;
+            float any = 0.0f;
+#pragma unroll
+            for (int m = 0; m < M; m++) any += seg[i][m];
+            // any > 0 whenever a sum prefix is > 0 (all terms are >= 0)
+            if (0.0f == any) continue;
+            if (vd.prio[i] < cp) {
+                cp = vd.prio[i];
+                n_at = 1;
+            } else if (vd.prio[i] == cp) {
+                n_at += 1;
+            }
+        }
+        if (0 == n_at) {
+            if (atten) set_m<M>(ad, air, add_air(get_m<M>(ad, air)));
+        } else {
+            float weight = 1.0f / ((float)n_at);
+            weight *= (0 == t || N - 1 == t) ? 0.5f : 1.0f;
+#pragma unroll
+            for (int i = 0; i < V; i++) {
+                if (trace[i] && vd.prio[i] == cp && vd.en[i] == 1) {
+                    const VolDev &vol = vd.vol[i];
+                    const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
+                    const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                                                    clamp_cell(bz, vol.nz));
+                    const float4 d0 = vol.dens[2 * cell], d1 = vol.dens[2 * cell + 1];
+                    float w[8];
+                    weights8(p[i][0] - bx, p[i][1] - by, p[i][2] - bz, w);
+                    const float rho = trilerp(d0, d1, w);
+#pragma unroll
+                    for (int m = 0; m < M; m++) ad[m] += (weight * rho) * seg[i][m];
+                }
+            }
+        }
+        alpha += step;
+    }
+    if (atten) {
+        float a = get_m<M>(ad, air);
+        a = (float)((double)a + (double)((L - maxA) / step) * DRR_AIR_DENSITY);
+        set_m<M>(ad, air, a);
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] *= step;
+#pragma unroll
+    for (int m = 0; m < M; m++) ad[m] /= 10.0f;
+}
+
+// Spectral Beer-Lambert sum (project_kernel.cu:637-646).
+template <int M>
+__device__ __forceinline__ void spectral(const SpectrumDev &sp, const float (&ad)[M], float &intensity,
+                                         float &photon_prob) {
+    float I = 0.0f, P = 0.0f;
+    for (int b = 0; b < sp.n_bins; b++) {
+        float e = 0.0f;
+#pragma unroll
+        for (int m = 0; m < M; m++) e += ad[m] * sp.mu[b * M + m];
+        const float p = expf(-1.f * e) * sp.pdf[b];
+        P += p;
+        I += sp.energies[b] * p;
+    }
+    intensity = I;
+    photon_prob = P;
+}
+
+// Solid angle of pixel (udx, vdx) (project_kernel.cu:14-133).
+__device__ __forceinline__ float solid_angle(const float *W, int udx, int vdx) {
+    float cx[4], cy[4], cz[4], cm[4];
+    const float cu_off[4] = {0.f, 1.f, 1.f, 0.f};
+    const float cv_off[4] = {0.f, 0.f, 1.f, 1.f};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float cu = udx + cu_off[c];
+        const float cv = vdx + cv_off[c];
+        cx[c] = cu * W[0] + cv * W[1] + W[2];
+        cy[c] = cu * W[3] + cv * W[4] + W[5];
+        cz[c] = cu * W[6] + cv * W[7] + W[8];
+        cm[c] = sqrtf((cx[c] * cx[c]) + (cy[c] * cy[c]) + (cz[c] * cz[c]));
+    }
+    const float x02 = (cy[0] * cz[2]) - (cz[0] * cy[2]);
+    const float y02 = (cz[0] * cx[2]) - (cx[0] * cz[2]);
+    const float z02 = (cx[0] * cy[2]) - (cy[0] * cx[2]);
+    const float d01 = (cx[0] * cx[1]) + (cy[0] * cy[1]) + (cz[0] * cz[1]);
+    const float d02 = (cx[0] * cx[2]) + (cy[0] * cy[2]) + (cz[0] * cz[2]);
+    const float d03 = (cx[0] * cx[3]) + (cy[0] * cy[3]) + (cz[0] * cz[3]);
+    const float d12 = (cx[1] * cx[2]) + (cy[1] * cy[2]) + (cz[1] * cz[2]);
+    const float d23 = (cx[2] * cx[3]) + (cy[2] * cy[3]) + (cz[2] * cz[3]);
+    const float n012 = fabsf((cx[1] * x02) + (cy[1] * y02) + (cz[1] * z02));
+    const float n023 = fabsf((cx[3] * x02) + (cy[3] * y02) + (cz[3] * z02));
+    const float den012 = (cm[0] * cm[1] * cm[2]) + (d01 * cm[2]) + (d02 * cm[1]) + (d12 * cm[0]);
+    const float den023 = (cm[0] * cm[2] * cm[3]) + (d02 * cm[3]) + (d03 * cm[2]) + (d23 * cm[0]);
+    float s012 = 2.f * atan2f(n012, den012);
+    if (s012 < 0.0f) s012 += 3.141592654f;
+    float s023 = 2.f * atan2f(n023, den023);
+    if (s023 < 0.0f) s023 += 3.141592654f;
+    return s012 + s023;
+}
+
+// Tile -> pixel mapping: a 64-lane wave covers an 8x8 pixel tile (a ray
+// packet whose samples fall in neighbouring cells at each depth).
+__device__ __forceinline__ void lane_pixel(int tile_x, int tile_y, int lane, int &udx, int &vdx) {
+    udx = tile_x * 8 + (lane & 7);
+    vdx = tile_y * 8 + (lane >> 3);
+}
+
+}  // namespace drr

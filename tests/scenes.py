@@ -1,0 +1,128 @@
+"""Seeded scenes shared by the parity tests, plus a thin runner that calls the
+HIP library through its C ABI."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from deepdrr_amd import phantoms, geo
+from deepdrr_amd import projector as P
+from deepdrr_amd._abi_types import Scene, View, make_view, view_to_dict
+from deepdrr_amd.material import attenuation_table
+from deepdrr_amd.spectral_data import spectrum_tables, synthetic_120kv
+from deepdrr_amd.volume import Volume
+
+
+@dataclass
+class Case:
+    name: str
+    volumes: list            # [(density [nz,ny,nx] f32, labels u8)]
+    view: View
+    scene: Scene
+    E: np.ndarray
+    pdf: np.ndarray
+    mu: np.ndarray
+    materials: List[str] = field(default_factory=list)
+
+    def scene_dict(self):
+        return dict(step=self.scene.step, n_materials=self.scene.n_materials,
+                    attenuate_outside=self.scene.attenuate_outside, air_index=self.scene.air_index)
+
+    def view_dict(self):
+        return view_to_dict(self.view)
+
+
+def make_case(name, vols: List[Volume], proj: geo.CameraProjection, step=0.1, atten=False,
+              priorities=None, max_ray_length=4080.0, spectrum=None) -> Case:
+    mats = set()
+    for v in vols:
+        mats |= set(v.materials[0].keys())
+    if atten:
+        mats.add("air")
+    mats = sorted(mats)
+    E, pdf = spectrum_tables(synthetic_120kv() if spectrum is None else spectrum)
+    mu = attenuation_table(mats, E)
+    prios = priorities if priorities is not None else [len(vols) - 1 - i for i in range(len(vols))]
+    view = P.build_view(vols, prios, proj, max_ray_length)
+    arrays = [P.volume_arrays(v, mats) for v in vols]
+    sc = Scene(step=step, n_materials=len(mats), attenuate_outside=1 if atten else 0,
+               air_index=mats.index("air") if atten else 0)
+    return Case(name, arrays, view, sc, E, pdf, mu, mats)
+
+
+def small_case(n=32, det=(24, 20), step=0.1, atten=False, seed=0) -> Case:
+    vol = phantoms.small_ct(n, seed)
+    pix = 0.8 * 128 / n * 2
+    dev = phantoms.ap_device(det[0], det[1], pix)
+    return make_case(f"small{n}_{det[0]}x{det[1]}_atten{int(atten)}", [vol], dev.get_camera_projection(),
+                     step=step, atten=atten)
+
+
+def oblique_case(n=24, det=(16, 16), seed=1, step=0.1) -> Case:
+    vol = phantoms.small_ct(n, seed)
+    rng = np.random.default_rng(seed)
+    (p, d, f), = phantoms.batch_poses(1, seed=seed, radius=0.2 * n)
+    proj = phantoms.pose_projection(det[0], det[1], 0.8 * 128 / n * 2, p, d, f)
+    return make_case(f"oblique{n}", [vol], proj, step=step)
+
+
+def multi_case(step=0.1, same_grid=False, atten=False, priorities=None) -> Case:
+    a = phantoms.small_ct(20, 0)
+    if same_grid:
+        # identical grid, different segmentation: exercises the shared label cache
+        hu = phantoms.small_ct_hu(20, 3)
+        b = Volume.from_hu(hu, origin=np.array(a.anatomical_from_IJK.t), spacing=(1, 1, 1),
+                           anatomical_coordinate_system="LPS")
+    else:
+        b = phantoms.uniform_volume((10, 12, 8), spacing=(1.5, 1.0, 2.0), density=1.9, material="bone")
+        b.place_center([3.0, 0.0, -2.0])
+    dev = phantoms.ap_device(18, 14, 3.0)
+    return make_case(f"multi_same{int(same_grid)}_atten{int(atten)}", [a, b], dev.get_camera_projection(),
+                     step=step, atten=atten, priorities=priorities)
+
+
+class GpuRunner:
+    """Owns a drr_ctx and volume handles for one Case."""
+
+    def __init__(self, case: Case):
+        import torch
+        from deepdrr_amd import _abi
+
+        self.torch = torch
+        self.abi = _abi
+        self.case = case
+        self.ctx = C.c_void_p()
+        _abi.call("drr_ctx_create", 0, C.byref(self.ctx))
+        _abi.call("drr_ctx_set_scene", self.ctx, C.byref(case.scene))
+        _abi.call("drr_ctx_set_spectrum", self.ctx, len(case.E), case.E.ctypes.data,
+                  case.pdf.ctypes.data, case.mu.ctypes.data)
+        self.vols = []
+        for d, l in case.volumes:
+            h = C.c_void_p()
+            nz, ny, nx = d.shape
+            _abi.call("drr_volume_create", self.ctx, d.ctypes.data, l.ctypes.data, nx, ny, nz, 0, C.byref(h))
+            self.vols.append(h)
+        self.varr = (C.c_void_p * len(self.vols))(*[v.value for v in self.vols])
+
+    def project(self, view: View = None, solid_angle=False):
+        torch = self.torch
+        view = view or self.case.view
+        H, W, M = view.height, view.width, self.case.scene.n_materials
+        dev = torch.device("cuda", 0)
+        I = torch.full((H, W), float("nan"), device=dev)
+        Pp = torch.full((H, W), float("nan"), device=dev)
+        AD = torch.full((M, H, W), float("nan"), device=dev)
+        SA = torch.full((H, W), float("nan"), device=dev) if solid_angle else None
+        self.abi.call("drr_project", self.ctx, self.varr, C.byref(view), self.abi.ptr(I), self.abi.ptr(Pp),
+                      self.abi.ptr(AD), self.abi.ptr(SA), self.abi.stream_ptr())
+        torch.cuda.synchronize()
+        out = (I.cpu().numpy(), Pp.cpu().numpy(), AD.cpu().numpy())
+        return out + ((SA.cpu().numpy(),) if solid_angle else ())
+
+    def close(self):
+        for v in self.vols:
+            self.abi.lib().drr_volume_destroy(v)
+        self.abi.lib().drr_ctx_destroy(self.ctx)
