@@ -103,7 +103,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["batched", "plain"], default="plain")
+    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--tuning", type=int, nargs=4, default=[8, 8, 4, 8],
+                    help="tile_w tile_h waves_per_block blocks_per_cu")
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--step-mm", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -135,28 +137,25 @@ def main():
     geom = workload.clinical_geometry_volume()
     ap_dev = phantoms.ap_device(1536, 1536, 0.194)
     ap_view = P.build_view([geom], [0], ap_dev.get_camera_projection(), 4 * 1020.0)
-    n_samples_ap = workload.count_samples(ap_view, args.step_mm)
+    n_samples_ap = workload.count_samples(ap_view, args.step_mm, stride=2)
 
     # ---- plain path (reference flow), always measured -----------------------
     plain_n = args.plain_views
-    views_plain = [ap_view] * plain_n
     log("plain warmup")
-    plain_views(torch, _abi, ctx, varr, views_plain[:1], dev, utils)
+    plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    _, kms = plain_views(torch, _abi, ctx, varr, views_plain, dev, utils)
+    _, kms_plain = plain_views(torch, _abi, ctx, varr, [ap_view] * plain_n, dev, utils)
     torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t
-    plain_vps = plain_n / plain_s
-    kernel_ms = float(np.mean(kms))
-    log(f"plain: {plain_vps:.3f} views/s, march kernel {kernel_ms:.2f} ms/view")
+    plain_vps = plain_n / (time.perf_counter() - t)
+    plain_kernel_ms = float(np.mean(kms_plain))
+    log(f"plain: {plain_vps:.3f} views/s, march kernel {plain_kernel_ms:.2f} ms/view")
 
     result = dict(metric="views/s (512x512x400 CT->1536^2 det) batched vs plain; adjoint ms/view",
                   unit="views/s", n_gpus=1, higher_is_better=True, scaling="weak", vs_baseline=None,
                   dtype="f32", data="synthetic")
-
+    steps, warm = args.steps, args.warmup
     if args.mode == "plain":
-        steps, warm = args.steps, args.warmup
         for _ in range(warm):
             plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
         torch.cuda.synchronize()
@@ -169,21 +168,52 @@ def main():
         dt = time.perf_counter() - t
         value = steps / dt
         kernel_ms = float(np.mean(kms))
-        ms_per_step = dt / steps * 1e3
         samples_per_launch = n_samples_ap
+        kernel_name = "k_project_v1<5,false>"
         config = {"workload": "plain forward, 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
                               "1536x1536 detector at 0.194 mm, AP view, step 0.1 mm, 20-bin 120 kV, neglog",
                   "views_per_step": 1, "mode": "plain"}
     else:
-        raise SystemExit("batched mode not built yet")
+        from deepdrr_amd.batch import BatchPlan
 
+        B = args.batch
+        poses = phantoms.batch_poses(B, seed=0)
+        views = [P.build_view([geom], [0], phantoms.pose_projection(1536, 1536, 0.194, p, d, f),
+                              4 * 1020.0) for p, d, f in poses]
+        log("counting in-volume samples of the batch")
+        samples_per_launch = sum(workload.count_samples(v, args.step_mm, stride=2) for v in views)
+        plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True, tuning=tuple(args.tuning))
+        log(f"batched warmup ({B} views per launch)")
+        for _ in range(warm):
+            plan.launch()
+            plan.fetch("images")
+        torch.cuda.synchronize()
+        kms = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            plan.launch()
+            plan.fetch("images")       # one D2H of the B neglog images
+            kms.append(plan.march_ms())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        value = B * steps / dt
+        kernel_ms = float(np.mean(kms))
+        kernel_name = "k_batch_march<5,false,8>"
+        plan.close()
+        config = {"workload": f"batched forward, {B} poses (100 mm ball, 30 deg cone, source fraction "
+                              "U(0.5,0.9)) of a 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
+                              "1536x1536 detector at 0.194 mm, step 0.1 mm, 20-bin 120 kV, device neglog, "
+                              "one persistent launch per step replayed from a hipGraph",
+                  "views_per_step": B, "mode": "batched", "tuning": list(args.tuning)}
+    ms_per_step = dt / steps * 1e3
     achieved = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
     result.update(value=value, steps=args.steps, warmup=args.warmup, ms_per_step=ms_per_step,
                   config=config,
-                  plain_views_per_s=plain_vps,
+                  plain_views_per_s=plain_vps, plain_kernel_ms=plain_kernel_ms,
+                  batched_over_plain=value / plain_vps,
                   roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                            "kernel": "k_project_v1<5,false>", "kernel_ms": kernel_ms,
+                            "kernel": kernel_name, "kernel_ms": kernel_ms,
                             "samples_per_launch": samples_per_launch,
                             "samples_per_s": samples_per_launch / (kernel_ms * 1e-3)})
     if not args.no_cpu:

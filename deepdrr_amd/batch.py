@@ -1,0 +1,124 @@
+"""Batched forward projection: many (volume, pose, detector) views in one
+persistent launch replayed from a hipGraph.
+
+This is the batched form of deepdrr's per-view loop
+(``deepdrr/projector/projector.py:679-707``): the same images, computed by one
+graph launch with the neglog epilogue on the device and one device-to-host
+copy at the end.  Every job views exactly one volume.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi, geo
+from ._abi_types import BatchTuning, Job
+
+DEFAULT_TUNING = (8, 8, 4, 8)  # tile_w, tile_h, waves_per_block, blocks_per_cu
+
+
+class BatchPlan:
+    """A captured batch over device-resident volumes.
+
+    ``ctx``: a drr_ctx handle; ``vol_handles``: drr_volume handles; ``views``:
+    ctypes ``View`` objects (one volume each) and ``volume_indices`` into
+    ``vol_handles``.  Outputs live in device tensors ``intensity``,
+    ``photon_prob`` and (with ``neglog``) ``images``, each image at
+    ``offsets[i]`` with shape ``shapes[i]``.
+    """
+
+    def __init__(self, ctx, vol_handles: Sequence, views: Sequence, volume_indices: Sequence[int],
+                 neglog: bool = True, tuning=None, device=None):
+        import torch
+
+        self.torch = torch
+        self.device = device or torch.device("cuda", 0)
+        n = len(views)
+        if n == 0:
+            raise ValueError("empty batch")
+        if len(volume_indices) != n:
+            raise ValueError("one volume index per view is needed")
+        self.shapes = [(v.height, v.width) for v in views]
+        sizes = [h * w for h, w in self.shapes]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        total = int(np.sum(sizes))
+        jobs = (Job * n)()
+        for i, (v, vi) in enumerate(zip(views, volume_indices)):
+            if v.n_volumes != 1:
+                raise ValueError("each batched view must contain exactly one volume")
+            jobs[i].view = v
+            jobs[i].volume_index = int(vi)
+            jobs[i].out_offset = int(self.offsets[i])
+        self.intensity = torch.empty(total, dtype=torch.float32, device=self.device)
+        self.photon_prob = torch.empty(total, dtype=torch.float32, device=self.device)
+        self.neglog = bool(neglog)
+        self.images = torch.empty(total, dtype=torch.float32, device=self.device) if neglog else None
+        t = BatchTuning(*(tuning or DEFAULT_TUNING))
+        self._varr = (C.c_void_p * len(vol_handles))(*[h.value if hasattr(h, "value") else h
+                                                         for h in vol_handles])
+        self._batch = C.c_void_p()
+        _abi.call("drr_batch_create", ctx, self._varr, len(vol_handles), jobs, n, C.byref(t),
+                  _abi.ptr(self.intensity), _abi.ptr(self.photon_prob), _abi.ptr(self.images),
+                  1 if neglog else 0, C.byref(self._batch))
+        self.n_views = n
+        self.total = total
+
+    def launch(self, stream=None) -> None:
+        s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        _abi.call("drr_batch_launch", self._batch, _abi.stream_ptr(s))
+
+    def march_ms(self) -> float:
+        ms = C.c_float()
+        _abi.call("drr_batch_last_march_ms", self._batch, C.byref(ms))
+        return ms.value
+
+    def fetch(self, which: str = "images") -> List[np.ndarray]:
+        buf = {"images": self.images, "intensity": self.intensity, "photon_prob": self.photon_prob}[which]
+        if buf is None:
+            raise ValueError(f"{which} was not requested when the batch was planned")
+        host = buf.cpu().numpy()
+        return [host[o:o + h * w].reshape(h, w) for o, (h, w) in zip(self.offsets, self.shapes)]
+
+    def __call__(self, which: Optional[str] = None):
+        self.launch()
+        out = self.fetch(which or ("images" if self.neglog else "intensity"))
+        if len(set(self.shapes)) == 1:
+            return np.stack(out)
+        return out
+
+    def close(self) -> None:
+        if self._batch:
+            _abi.lib().drr_batch_destroy(self._batch)
+            self._batch = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_for_projector(projector, projections: Sequence[geo.CameraProjection],
+                       volume_indices: Optional[Sequence[int]] = None, neglog: Optional[bool] = None,
+                       tuning=None) -> BatchPlan:
+    """Batch plan over an initialised ``Projector``'s volumes."""
+    from .projector import build_view
+
+    if not projector.initialized:
+        raise RuntimeError("Projector has not been initialized.")
+    if volume_indices is None:
+        if len(projector.volumes) != 1:
+            raise ValueError("volume_indices is required when the projector holds several volumes")
+        volume_indices = [0] * len(projections)
+    if not hasattr(projector, "max_ray_length"):
+        projector.max_ray_length = projector.source_to_detector_distance * 4
+    views = []
+    sdd = projector.source_to_detector_distance
+    for proj, vi in zip(projections, volume_indices):
+        views.append(build_view([projector.volumes[vi]], [projector.priorities[vi]], proj,
+                                projector.max_ray_length, sdd if sdd > 0 else 1.0))
+    return BatchPlan(projector._ctx, projector._vols, views, volume_indices,
+                     neglog=projector.neglog if neglog is None else neglog, tuning=tuning,
+                     device=projector._torch_device)

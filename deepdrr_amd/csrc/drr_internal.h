@@ -72,12 +72,42 @@ struct ViewDev {
     VolDev vol[DRR_MAX_VOLUMES];
 };
 
+// One view of a single volume: the uniforms of the single-volume march.
+struct Geo1 {
+    float W[9];
+    float mrl;
+    int width, height;
+    float A[12];
+    float s[3];
+    float lo[3];
+    float hi[3];
+    int en;
+};
+
+inline void view_to_geo1(const drr_view_t &v, Geo1 &g) {
+    for (int i = 0; i < 9; i++) g.W[i] = v.world_from_index[i];
+    g.mrl = v.max_ray_length;
+    g.width = v.width;
+    g.height = v.height;
+    for (int i = 0; i < 12; i++) g.A[i] = v.vol[0].ijk_from_world[i];
+    for (int i = 0; i < 3; i++) {
+        g.s[i] = v.vol[0].source_ijk[i];
+        g.lo[i] = v.vol[0].bounds_lo[i];
+        g.hi[i] = v.vol[0].bounds_hi[i];
+    }
+    g.en = v.vol[0].enabled;
+}
+
 struct SpectrumDev {
     const float *energies;
     const float *pdf;
     const float *mu;
     int n_bins;
 };
+
+inline VolDev vol_to_dev(const drr_volume *v) {
+    return VolDev{v->dens, v->labels, v->nx, v->ny, v->nz};
+}
 
 inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, ViewDev &d) {
     for (int i = 0; i < 9; i++) d.W[i] = v.world_from_index[i];

@@ -73,27 +73,27 @@ struct OutDev {
 };
 
 template <int M, bool ATTEN>
-__global__ __launch_bounds__(256) void k_project_v1(ViewDev vd, SpectrumDev sp, float step, int air,
-                                                    OutDev out, int tiles16_x) {
+__global__ __launch_bounds__(256) void k_project_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step,
+                                                    int air, OutDev out, int tiles16_x) {
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int bx = lb % tiles16_x, by = lb / tiles16_x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int udx, vdx;
     lane_pixel(bx * 2 + (wave & 1), by * 2 + (wave >> 1), lane, udx, vdx);
-    if (udx >= vd.width || vdx >= vd.height) return;
+    if (udx >= g.width || vdx >= g.height) return;
     float ad[M];
-    march_v1<M, ATTEN>(vd, vd.vol[0], udx, vdx, step, air, ad);
+    march_v1<M, ATTEN>(g, vol, udx, vdx, step, air, ad);
     float I, P;
     spectral<M>(sp, ad, I, P);
-    const int64_t npix = (int64_t)vd.width * vd.height;
-    const int64_t idx = (int64_t)vdx * vd.width + udx;
+    const int64_t npix = (int64_t)g.width * g.height;
+    const int64_t idx = (int64_t)vdx * g.width + udx;
     out.intensity[idx] = I;
     out.photon_prob[idx] = P;
     if (out.area_density) {
 #pragma unroll
         for (int m = 0; m < M; m++) out.area_density[m * npix + idx] = ad[m];
     }
-    if (out.solid_angle) out.solid_angle[idx] = solid_angle(vd.W, udx, vdx);
+    if (out.solid_angle) out.solid_angle[idx] = solid_angle(g.W, udx, vdx);
 }
 
 template <int V, int M>
@@ -122,12 +122,12 @@ __global__ __launch_bounds__(256) void k_project_multi(ViewDev vd, SpectrumDev s
 }
 
 template <int M>
-static void launch_v1(bool atten, dim3 g, dim3 b, hipStream_t st, const ViewDev &vd,
+static void launch_v1(bool atten, dim3 g, dim3 b, hipStream_t st, const Geo1 &geo, const VolDev &vol,
                       const SpectrumDev &sp, float step, int air, const OutDev &o, int tx) {
     if (atten)
-        k_project_v1<M, true><<<g, b, 0, st>>>(vd, sp, step, air, o, tx);
+        k_project_v1<M, true><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
     else
-        k_project_v1<M, false><<<g, b, 0, st>>>(vd, sp, step, air, o, tx);
+        k_project_v1<M, false><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
 }
 
 template <int V>
@@ -154,6 +154,9 @@ extern "C" int drr_project(drr_ctx *ctx, const drr_volume *const *vols, const dr
     hipStream_t st = (hipStream_t)stream;
     ViewDev vd;
     view_to_dev(*view, vols, vd);
+    Geo1 geo;
+    view_to_geo1(*view, geo);
+    const VolDev vol0 = vol_to_dev(vols[0]);
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
     OutDev o{intensity, photon_prob, area_density, solid_angle_out};
     const drr_scene_t &sc = ctx->scene;
@@ -163,7 +166,7 @@ extern "C" int drr_project(drr_ctx *ctx, const drr_volume *const *vols, const dr
     if (view->n_volumes == 1) {
         switch (sc.n_materials) {
 #define DRR_CASE(MM) \
-    case MM: launch_v1<MM>(sc.attenuate_outside != 0, grid, block, st, vd, sp, sc.step, sc.air_index, o, tx); break;
+    case MM: launch_v1<MM>(sc.attenuate_outside != 0, grid, block, st, geo, vol0, sp, sc.step, sc.air_index, o, tx); break;
             DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
 #undef DRR_CASE
         }

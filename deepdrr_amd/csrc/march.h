@@ -76,8 +76,10 @@ __device__ __forceinline__ int clamp_cell(float b, int n) {
     return (int)c;
 }
 
-__device__ __forceinline__ int64_t cell_index(const VolDev &v, int cx, int cy, int cz) {
-    return ((int64_t)(cz + 1) * (v.ny + 1) + (cy + 1)) * (int64_t)(v.nx + 1) + (cx + 1);
+// Cells of one volume are < 2^31 (checked at upload), so 32-bit indices suffice.
+__device__ __forceinline__ uint32_t cell_index(const VolDev &v, int cx, int cy, int cz) {
+    return ((uint32_t)(cz + 1) * (uint32_t)(v.ny + 1) + (uint32_t)(cy + 1)) * (uint32_t)(v.nx + 1) +
+           (uint32_t)(cx + 1);
 }
 
 __device__ __forceinline__ int clamp_voxel(float f, int n) {
@@ -96,7 +98,7 @@ __device__ __forceinline__ uint32_t label_point(const VolDev &v, float x, float 
 // Labels of the 8 corners for sample p with base b: L[clamp(floor(p + d))]
 // (point sampling at p + d, project_kernel.cu:421-425).  Packed 4 per word.
 __device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py, float pz,
-                                             float bx, float by, float bz, int64_t cell,
+                                             float bx, float by, float bz, uint32_t cell,
                                              uint32_t &l0, uint32_t &l1) {
     const bool regular = (floorf(px + 1.0f) == bx + 1.0f) && (floorf(py + 1.0f) == by + 1.0f) &&
                          (floorf(pz + 1.0f) == bz + 1.0f);
@@ -132,6 +134,19 @@ __device__ __forceinline__ uint32_t corner_label(uint32_t l0, uint32_t l1, int d
     return ((d < 4 ? l0 : l1) >> (8 * (d & 3))) & 0xffu;
 }
 
+// seg[m] = sum over corners (dz, dy, dx order) of w[d] where label == m, for a
+// cell whose labels differ.
+template <int M>
+__device__ __forceinline__ void mixed_fractions(uint32_t l0, uint32_t l1, const float w[8], float seg[M]) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 8; d++) s += (corner_label(l0, l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+        seg[m] = s;
+    }
+}
+
 // seg[m] = sum over corners (dz, dy, dx order) of w[d] where label == m.
 template <int M>
 __device__ __forceinline__ void material_fractions(uint32_t l0, uint32_t l1, const float w[8],
@@ -153,9 +168,10 @@ __device__ __forceinline__ void material_fractions(uint32_t l0, uint32_t l1, con
     }
 }
 
+// Density at the sample: corner densities weighted by the same exact FP32
+// weights, summed in corner order starting from the first product.
 __device__ __forceinline__ float trilerp(const float4 &d0, const float4 &d1, const float w[8]) {
-    float rho = 0.0f;
-    rho += w[0] * d0.x;
+    float rho = w[0] * d0.x;
     rho += w[1] * d0.y;
     rho += w[2] * d0.z;
     rho += w[3] * d0.w;
@@ -180,6 +196,31 @@ __device__ __forceinline__ void set_m(float (&a)[M], int idx, float val) {
     for (int m = 0; m < M; m++) a[m] = (m == idx) ? val : a[m];
 }
 
+// Cached trilinear cell of the ray: base, 8 labels, 8 densities and whether
+// the labels agree.  `load` reads it for sample p with base b.
+struct Cell {
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    bool have = false;
+    bool uni = false;
+    uint32_t m0 = 0;
+    uint32_t l0 = 0, l1 = 0;
+    float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), d1 = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    __device__ __forceinline__ void load(const VolDev &vol, float px, float py, float pz, float x,
+                                         float y, float z) {
+        const uint32_t cell = cell_index(vol, clamp_cell(x, vol.nx), clamp_cell(y, vol.ny), clamp_cell(z, vol.nz));
+        fetch_labels(vol, px, py, pz, x, y, z, cell, l0, l1);
+        d0 = vol.dens[2 * cell];
+        d1 = vol.dens[2 * cell + 1];
+        m0 = l0 & 0xffu;
+        uni = (l0 == l1) && (l0 == m0 * 0x01010101u);
+        bx = x;
+        by = y;
+        bz = z;
+        have = true;
+    }
+};
+
 // Per-ray state of the single-volume march, filled by march_setup_v1.
 struct RayV1 {
     float rx, ry, rz, L;
@@ -190,18 +231,16 @@ struct RayV1 {
     bool trace;
 };
 
-__device__ __forceinline__ void march_setup_v1(const ViewDev &vd, int udx, int vdx, float step,
-                                               RayV1 &r) {
-    ray_dir(vd.W, udx, vdx, r.rx, r.ry, r.rz, r.L);
+__device__ __forceinline__ void march_setup_v1(const Geo1 &g, int udx, int vdx, float step, RayV1 &r) {
+    ray_dir(g.W, udx, vdx, r.rx, r.ry, r.rz, r.L);
     r.minA = r.L;
     r.maxA = 0.0f;
     r.trace = false;
     r.ri[0] = r.ri[1] = r.ri[2] = 0.0f;
     r.mnv = 0.0f;
     r.mxv = 0.0f;
-    if (vd.en[0]) {
-        r.trace = slab(vd.A[0], vd.s[0], vd.lo[0], vd.hi[0], vd.max_ray_length, r.rx, r.ry, r.rz,
-                       r.ri, r.mnv, r.mxv);
+    if (g.en) {
+        r.trace = slab(g.A, g.s, g.lo, g.hi, g.mrl, r.rx, r.ry, r.rz, r.ri, r.mnv, r.mxv);
         if (r.trace) {
             r.minA = fminf(r.minA, r.mnv);
             r.maxA = fmaxf(r.maxA, r.mxv);
@@ -213,10 +252,10 @@ __device__ __forceinline__ void march_setup_v1(const ViewDev &vd, int udx, int v
 // Single-volume march (project_kernel.cu:334-584 with NUM_VOLUMES == 1).
 // On return ad[] holds the final area densities in g/cm^2.
 template <int M, bool ATTEN>
-__device__ __forceinline__ void march_v1(const ViewDev &vd, const VolDev &vol, int udx, int vdx,
+__device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int udx, int vdx,
                                          float step, int air, float (&ad)[M]) {
     RayV1 r;
-    march_setup_v1(vd, udx, vdx, step, r);
+    march_setup_v1(g, udx, vdx, step, r);
 #pragma unroll
     for (int m = 0; m < M; m++) ad[m] = 0.0f;
     float acc = 0.0f;  // air accumulator while no volume is sampled
@@ -226,7 +265,7 @@ __device__ __forceinline__ void march_v1(const ViewDev &vd, const VolDev &vol, i
     float alpha = r.minA;
     int t = 0;
     if (r.trace) {
-        const float sx = vd.s[0][0], sy = vd.s[0][1], sz = vd.s[0][2];
+        const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
         const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
         // Phase 1: steps well before the volume: only the alpha sequence.
         const float rdom = fmaxf(fabsf(rix), fmaxf(fabsf(riy), fabsf(riz)));
@@ -244,60 +283,57 @@ __device__ __forceinline__ void march_v1(const ViewDev &vd, const VolDev &vol, i
             }
         }
         // Phase 2: approach window, label cache tracked as the reference does.
-        float pbx = 0.0f, pby = 0.0f, pbz = 0.0f;
-        bool have = false;
-        uint32_t l0 = 0, l1 = 0;
-        float4 d0 = make_float4(0, 0, 0, 0), d1 = d0;
+        Cell c;
         while (t < N && alpha < r.mnv) {
             const float px = sx + alpha * rix - 1.0f;
             const float py = sy + alpha * riy - 1.0f;
             const float pz = sz + alpha * riz - 1.0f;
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-            if (!have || bx != pbx || by != pby || bz != pbz) {
-                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
-                                                clamp_cell(bz, vol.nz));
-                fetch_labels(vol, px, py, pz, bx, by, bz, cell, l0, l1);
-                d0 = vol.dens[2 * cell];
-                d1 = vol.dens[2 * cell + 1];
-                pbx = bx;
-                pby = by;
-                pbz = bz;
-                have = true;
-            }
+            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) c.load(vol, px, py, pz, bx, by, bz);
             if (ATTEN) acc = add_air(acc);
             alpha += step;
             ++t;
         }
         if (ATTEN) set_m<M>(ad, air, acc);
         // Phase 3: inside [minAlpha_vol, maxAlpha_vol]: sample and accumulate.
+        // While the ray is in a cell whose 8 labels agree (material m0), only
+        // ad[m0] changes; it is kept in `cur` and written back on leaving, so
+        // every FP32 addition is the one the reference performs.
+        float cur = 0.0f;
+        int cm = -1;
+        if (c.have && c.uni) {
+            cm = (int)c.m0;
+            cur = get_m<M>(ad, cm);
+        }
         while (t < N && alpha <= r.mxv) {
             const float px = sx + alpha * rix - 1.0f;
             const float py = sy + alpha * riy - 1.0f;
             const float pz = sz + alpha * riz - 1.0f;
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-            if (!have || bx != pbx || by != pby || bz != pbz) {
-                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
-                                                clamp_cell(bz, vol.nz));
-                fetch_labels(vol, px, py, pz, bx, by, bz, cell, l0, l1);
-                d0 = vol.dens[2 * cell];
-                d1 = vol.dens[2 * cell + 1];
-                pbx = bx;
-                pby = by;
-                pbz = bz;
-                have = true;
+            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
+                if (cm >= 0) set_m<M>(ad, cm, cur);
+                c.load(vol, px, py, pz, bx, by, bz);
+                cm = c.uni ? (int)c.m0 : -1;
+                cur = get_m<M>(ad, cm);
             }
             float w[8];
             weights8(px - bx, py - by, pz - bz, w);
-            float seg[M];
-            material_fractions<M>(l0, l1, w, seg);
-            const float rho = trilerp(d0, d1, w);
+            const float rho = trilerp(c.d0, c.d1, w);
             const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
             const float x = wt * rho;
+            const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
+            // uniform cell: seg[m0] = ssum, all other fractions 0
+            cur += c.uni ? x * ssum : 0.0f;
+            if (!c.uni) {
+                float seg[M];
+                mixed_fractions<M>(c.l0, c.l1, w, seg);
 #pragma unroll
-            for (int m = 0; m < M; m++) ad[m] += x * seg[m];
+                for (int m = 0; m < M; m++) ad[m] += x * seg[m];
+            }
             alpha += step;
             ++t;
         }
+        if (cm >= 0) set_m<M>(ad, cm, cur);
         if (ATTEN) acc = get_m<M>(ad, air);
     }
     // Phase 4: the rest of the steps sample nothing.
@@ -361,7 +397,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
             p[i][2] = vd.s[i][2] + alpha * ri[i][2] - 1.0f;
             const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
             if (!have || bx != pbx || by != pby || bz != pbz) {
-                const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
                                                 clamp_cell(bz, vol.nz));
                 fetch_labels(vol, p[i][0], p[i][1], p[i][2], bx, by, bz, cell, l0, l1);
                 pbx = bx;
@@ -400,7 +436,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
                 if (trace[i] && vd.prio[i] == cp && vd.en[i] == 1) {
                     const VolDev &vol = vd.vol[i];
                     const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
-                    const int64_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
+                    const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
                                                     clamp_cell(bz, vol.nz));
                     const float4 d0 = vol.dens[2 * cell], d1 = vol.dens[2 * cell + 1];
                     float w[8];

@@ -193,6 +193,9 @@ static int check_dims(int nx, int ny, int nz) {
     if (nx < 1 || ny < 1 || nz < 1) return drr_set_error(DRR_ERR_INVALID, "drr_volume: dimensions must be >= 1");
     if (nx > 16384 || ny > 16384 || nz > 16384)
         return drr_set_error(DRR_ERR_INVALID, "drr_volume: dimensions must be <= 16384");
+    // the march indexes cells (and 2 float4 per cell) with 32-bit integers
+    if ((int64_t)(nx + 1) * (ny + 1) * (nz + 1) >= ((int64_t)1 << 31))
+        return drr_set_error(DRR_ERR_INVALID, "drr_volume: (nx+1)(ny+1)(nz+1) must be < 2^31 cells");
     return DRR_OK;
 }
 

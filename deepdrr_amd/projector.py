@@ -354,6 +354,51 @@ class Projector:
         return self.project(*args, **kwargs)
 
     # -- extras beyond the reference ------------------------------------------
+    def project_batch(self, *camera_projections: geo.CameraProjection, volume_indices=None,
+                      tuning=None):
+        """All views in one persistent launch (hipGraph), device neglog, one D2H copy.
+
+        Returns what ``project`` returns for the same views ([H, W] for one view,
+        [N, H, W] for equal sizes, else a list).  Each view renders one volume
+        (``volume_indices``; required when the projector holds several).
+        """
+        from .batch import plan_for_projector
+
+        camera_projections = self._prepare_project(camera_projections)
+        host_epilogue = self.add_noise or self.intensity_upper_bound is not None or self.scatter_num > 0
+        plan = plan_for_projector(self, camera_projections, volume_indices,
+                                  neglog=self.neglog and not host_epilogue, tuning=tuning)
+        try:
+            plan.launch()
+            if host_epilogue:
+                images = plan.fetch("intensity")
+                photon_prob = plan.fetch("photon_prob")
+                out = []
+                for k, (img, pp) in enumerate(zip(images, photon_prob)):
+                    x = img[None]
+                    if self.scatter_num > 0:
+                        from .mc import scatter_for_projector
+
+                        x = x + scatter_for_projector(self, [camera_projections[k]])
+                    if self.add_noise:
+                        x = utils.add_noise(x, pp[None], self.photon_count)
+                    if self.intensity_upper_bound is not None:
+                        x = np.clip(x, None, self.intensity_upper_bound)
+                    out.append(x[0])
+                if self.neglog and len({o.shape for o in out}) == 1:
+                    out = list(utils.neglog(np.stack(out)))
+                elif self.neglog:
+                    out = [utils.neglog(o) for o in out]
+            else:
+                out = plan.fetch("images" if self.neglog else "intensity")
+        finally:
+            plan.close()
+        if len(out) == 1:
+            return out[0]
+        if len({o.shape for o in out}) == 1:
+            return np.stack(out)
+        return out
+
     def project_area_density(self, proj: geo.CameraProjection):
         """Per-material area densities [M, H, W] (g/cm^2), intensity and photon_prob of one view."""
         import torch

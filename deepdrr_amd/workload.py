@@ -23,12 +23,14 @@ CLINICAL_CLASSES = ("air", "lung", "soft tissue", "bone", "titanium")
 CLINICAL_UPPER_HU = (-900.0, -200.0, 300.0, 3000.0, 1.0e9)
 
 
-def ray_intervals(view: View):
-    """Per-ray [minAlpha_vol, maxAlpha_vol] and ray length of a single-volume view (FP64)."""
+def ray_intervals(view: View, stride: int = 1):
+    """Per-ray [minAlpha_vol, maxAlpha_vol] and ray length of a single-volume view
+    (FP64), for every ``stride``-th pixel along u and v."""
     F32 = np.float32
     W = np.array(view.world_from_index[:], F32)
     w, h = view.width, view.height
-    vv, uu = np.meshgrid(np.arange(h, dtype=F32), np.arange(w, dtype=F32), indexing="ij")
+    vv, uu = np.meshgrid(np.arange(0, h, stride, dtype=F32), np.arange(0, w, stride, dtype=F32),
+                         indexing="ij")
     u = uu.reshape(-1) + F32(0.5)
     v = vv.reshape(-1) + F32(0.5)
     rx = (u * W[0] + v * W[1]) + W[2]
@@ -54,16 +56,18 @@ def ray_intervals(view: View):
     return mn, mx, L.astype(np.float64)
 
 
-def count_samples(view: View, step: float) -> int:
-    """Closed-form count of in-volume samples of one single-volume view."""
-    mn, mx, L = ray_intervals(view)
+def count_samples(view: View, step: float, stride: int = 1) -> int:
+    """Closed-form count of in-volume samples of one single-volume view; with
+    ``stride`` > 1 every stride-th pixel is counted and the sum scaled up."""
+    mn, mx, L = ray_intervals(view, stride)
     minA = np.minimum(L, mn)
     maxA = np.maximum(0.0, mx)
     N = np.ceil((maxA - minA) / step)
     t0 = np.maximum(0.0, np.ceil((mn - minA) / step))
     t1 = np.minimum(N - 1, np.floor((mx - minA) / step))
     n = np.where(mx >= mn, np.maximum(0.0, t1 - t0 + 1), 0.0)
-    return int(n.sum())
+    scale = (view.width * view.height) / n.size
+    return int(round(n.sum() * scale))
 
 
 def clinical_label_map(materials):

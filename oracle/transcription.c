@@ -25,7 +25,8 @@
  *   labels  (point, clamp): corner d of the cell is L[clamp(floor(p + d))]
  *   density (linear, clamp, fetched at p + 0.5): corners clamp(floor(p)) and
  *           clamp(floor(p) + 1) weighted by the same exact FP32 weights as the
- *           labels, summed in (dz, dy, dx) order.  (NVIDIA's texture unit uses
+ *           labels, summed in (dz, dy, dx) order starting from the first
+ *           product.  (NVIDIA's texture unit uses
  *           8-bit fixed-point weights; exact weights are this engine's
  *           definition.)
  *   The label cache starts empty (the reference's first fetch is skipped if
@@ -232,8 +233,8 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
                     w[5] = fx * fyi * fz;
                     w[6] = fxi * fy * fz;
                     w[7] = fx * fy * fz;
-                    float rho = 0.0f;
-                    for (int d = 0; d < 8; d++) {
+                    float rho = w[0] * density_at(&vols[vi], ix, iy, iz);
+                    for (int d = 1; d < 8; d++) {
                         int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
                         rho += w[d] * density_at(&vols[vi], ix + dx, iy + dy, iz + dz);
                     }
