@@ -57,7 +57,7 @@ __global__ void k_batch_reset(int *counters) {
 }
 
 template <int M, bool ATTEN, int TW>
-__global__ __launch_bounds__(256) void k_batch_march(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M + (ATTEN ? 1 : 0)>::waves))) void k_batch_march(const JobDev *__restrict__ jobs,
                                                      const VolDev *__restrict__ vols,
                                                      const int2 *__restrict__ items,
                                                      const int *__restrict__ seg, int *counters,

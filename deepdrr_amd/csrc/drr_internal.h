@@ -25,6 +25,11 @@ struct drr_volume_s {
     int64_t n_cells = 0;
     float4 *dens = nullptr;     // 2 float4 per cell
     uint2 *labels = nullptr;    // 8 bytes per cell
+    // voxel-order copies [nz][ny][nx], made on first use by the Monte-Carlo
+    // path (nearest-voxel lookups read 5 bytes instead of a 40-byte cell)
+    float *lin_dens = nullptr;
+    uint8_t *lin_lab = nullptr;
+    float max_density = 0.0f;
 };
 
 struct drr_ctx_s {
@@ -138,6 +143,9 @@ inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, View
 
 // Validation shared by the entry points.
 int drr_check_view(const drr_ctx *ctx, const drr_volume *const *vols, const drr_view_t *view);
+
+// Make vol->lin_dens / lin_lab / max_density if absent (runtime.hip).
+int drr_volume_ensure_linear(drr_volume *vol);
 
 // Launchers implemented in forward.hip (used by batch/adjoint code).
 int drr_launch_expand(const float *density, const uint8_t *labels, int nx, int ny, int nz,

@@ -73,7 +73,7 @@ struct OutDev {
 };
 
 template <int M, bool ATTEN>
-__global__ __launch_bounds__(256) void k_project_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M>::waves))) void k_project_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step,
                                                     int air, OutDev out, int tiles16_x) {
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int bx = lb % tiles16_x, by = lb / tiles16_x;

@@ -221,6 +221,13 @@ struct Cell {
     }
 };
 
+// Occupancy target of the march kernels: up to 5 materials the march fits in
+// 64 VGPRs (8 waves per SIMD) without spills; more materials get more registers.
+template <int M>
+struct MarchOcc {
+    static constexpr int waves = M <= 5 ? 8 : 5;
+};
+
 // Per-ray state of the single-volume march, filled by march_setup_v1.
 struct RayV1 {
     float rx, ry, rz, L;

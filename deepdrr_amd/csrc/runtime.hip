@@ -313,6 +313,8 @@ int drr_volume_destroy(drr_volume *vol) {
     DeviceGuard g(vol->ctx ? vol->ctx->device : 0);
     (void)hipFree(vol->dens);
     (void)hipFree(vol->labels);
+    (void)hipFree(vol->lin_dens);
+    (void)hipFree(vol->lin_lab);
     delete vol;
     return DRR_OK;
 }
@@ -330,6 +332,40 @@ __global__ void k_unexpand(const float4 *dens, const uint2 *labels, int nx, int 
         out_l[i] = (uint8_t)(labels[cell].x & 0xffu);
     }
 }
+
+}  // extern "C"
+
+__global__ void k_max_density(const float *d, int64_t n, unsigned int *out) {
+    float m = 0.0f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, d[i]);
+    atomicMax(out, __float_as_uint(m));  // non-negative floats order like their bits
+}
+
+int drr_volume_ensure_linear(drr_volume *vol) {
+    if (vol->lin_dens) return DRR_OK;
+    const int64_t n = (int64_t)vol->nx * vol->ny * vol->nz;
+    DRR_HIP_CHECK(hipMalloc(&vol->lin_dens, sizeof(float) * n));
+    DRR_HIP_CHECK(hipMalloc(&vol->lin_lab, n));
+    k_unexpand<<<2048, 256, 0, 0>>>(vol->dens, vol->labels, vol->nx, vol->ny, vol->nz, vol->lin_dens,
+                                    vol->lin_lab);
+    DRR_HIP_CHECK(hipGetLastError());
+    unsigned int *d_max = nullptr;
+    DRR_HIP_CHECK(hipMalloc(&d_max, sizeof(unsigned int)));
+    DRR_HIP_CHECK(hipMemset(d_max, 0, sizeof(unsigned int)));
+    k_max_density<<<1024, 256, 0, 0>>>(vol->lin_dens, n, d_max);
+    DRR_HIP_CHECK(hipGetLastError());
+    unsigned int bits = 0;
+    DRR_HIP_CHECK(hipMemcpy(&bits, d_max, sizeof(unsigned int), hipMemcpyDeviceToHost));
+    (void)hipFree(d_max);
+    float mx;
+    std::memcpy(&mx, &bits, sizeof(float));
+    vol->max_density = mx;
+    return DRR_OK;
+}
+
+extern "C" {
 
 int drr_volume_download(drr_volume *vol, float *density, uint8_t *labels) {
     if (!vol || !density || !labels) return drr_set_error(DRR_ERR_INVALID, "drr_volume_download: NULL argument");

@@ -175,17 +175,23 @@ int drr_add_noise(drr_ctx *ctx, const float *intensity, const float *photon_prob
 
 /* ---- adjoint ----------------------------------------------------------------- */
 
-/* Backward of intensity for one view of a single volume:
- *   grad_density (device float32 [nz][ny][nx], accumulated atomically) +=
- *     d(sum grad_intensity * intensity) / d(density)
- *   grad_pose (device float32[6], accumulated) += d(...)/d(t_x, t_y, t_z, w_x, w_y, w_z)
- * for the volume pose perturbation x -> R(w) (x - c) + c + t in world
- * coordinates, c = pose_center_world (3 floats, host).  grad_pose may be NULL.
- * The per-ray march is recomputed (no stored paths). */
+/* Backward of one view of a single volume (the reference is not
+ * differentiable, README.md:47; this is the engine's adjoint of drr_project).
+ * With L = sum(grad_intensity * intensity + grad_photon_prob * photon_prob)
+ * (either gradient image may be NULL; device float32 [height][width]):
+ *   grad_density (device float32 [nz][ny][nx]) += dL/d(density), scattered
+ *     atomically with the trilinear weights of every sample;
+ *   grad_pose (device float32[6], may be NULL) += dL/d(t_x, t_y, t_z, w_x, w_y, w_z)
+ *     for the rigid volume perturbation x -> R(w) (x - c) + c + t applied in
+ *     world coordinates, c = pose_center_world, evaluated at w = 0, t = 0.
+ * source_world and pose_center_world are 3 host floats.  The per-ray march
+ * is recomputed twice (area densities, then gradients); nothing is stored
+ * per sample.  Boundary terms of the step count and of the volume entry/exit
+ * (piecewise constant in the pose) are not differentiated. */
 int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t *view,
-                const float *grad_intensity, const float *pose_center_world,
-                const float *world_from_ijk, float *grad_density, float *grad_pose,
-                void *stream);
+                const float *grad_intensity, const float *grad_photon_prob,
+                const float *source_world, const float *pose_center_world,
+                float *grad_density, float *grad_pose, void *stream);
 
 /* ---- Monte-Carlo scatter ----------------------------------------------------- */
 
@@ -216,14 +222,17 @@ typedef struct {
 } drr_mc_params_t;
 
 /* Run histories [first_history, first_history + n_histories) with RANECU
- * per-history streams (seed `seed`).  image: device float32
- * [4][H][W] energy (keV) deposited on the detector, split as
- * [primary, single Compton, single Rayleigh, multiple] (test_mcgpu.py:198-235
- * column order).  image2 (optional) receives the per-pixel sum of squares
- * for variance estimates.  Accumulates into image/image2. */
+ * per-history streams (seed `seed`; history h always draws the same
+ * numbers).  image: device float32 [4][H][W] photon energy (keV) reaching
+ * each detector pixel, split as [primary, single Compton, single Rayleigh,
+ * multiple] (the column order of MC-GPU's output, test_mcgpu.py:198-235).
+ * image2 (optional) receives the per-pixel sums of squared energies (for
+ * variance estimates); n_steps (optional, device int64) receives the number
+ * of Woodcock flight steps.  All three accumulate.  The volume is sampled at
+ * voxel centres (nearest voxel), IJK box [-0.5, n-0.5]. */
 int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_params_t *params,
                    int64_t first_history, int64_t n_histories, uint64_t seed,
-                   float *image, float *image2, void *stream);
+                   float *image, float *image2, int64_t *n_steps, void *stream);
 
 /* ---- timing helper ---------------------------------------------------------- */
 

@@ -78,3 +78,46 @@ def solid_angle(world_from_index, width: int, height: int) -> np.ndarray:
     out = np.zeros((height, width), np.float32)
     lib().ora_solid_angle(w.ctypes.data, width, height, out.ctypes.data)
     return out
+
+
+def _mc_lib():
+    l = lib()
+    if not getattr(l, "_mc_bound", False):
+        from deepdrr_amd._abi_types import MCParams
+
+        l.ora_mc.restype = C.c_long
+        l.ora_mc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(MCParams),
+                             C.c_int64, C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
+        l.ora_sample_compton.restype = None
+        l.ora_sample_compton.argtypes = [C.c_float, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p]
+        l.ora_sample_rayleigh.restype = None
+        l.ora_sample_rayleigh.argtypes = [C.c_int, C.c_uint64, C.c_void_p]
+        l._mc_bound = True
+    return l
+
+
+def mc(density_kji, labels_kji, params, n_histories, seed=12345, first=0, n_threads=None):
+    """CPU Monte Carlo: returns img [4,H,W], img2 [4,H,W] (float64) and the step count."""
+    d = np.ascontiguousarray(density_kji, np.float32)
+    l = np.ascontiguousarray(labels_kji, np.uint8)
+    nz, ny, nx = d.shape
+    H, W = params.det_height, params.det_width
+    img = np.zeros((4, H, W), np.float64)
+    img2 = np.zeros((4, H, W), np.float64)
+    steps = _mc_lib().ora_mc(d.ctypes.data, l.ctypes.data, nx, ny, nz, C.byref(params), first,
+                             n_histories, C.c_uint64(seed), img.ctypes.data, img2.ctypes.data,
+                             int(n_threads or default_threads()))
+    return img, img2, int(steps)
+
+
+def sample_compton(E, n, seed=1):
+    c = np.zeros(n, np.float32)
+    e = np.zeros(n, np.float32)
+    _mc_lib().ora_sample_compton(float(E), n, C.c_uint64(seed), c.ctypes.data, e.ctypes.data)
+    return c, e
+
+
+def sample_rayleigh(n, seed=1):
+    c = np.zeros(n, np.float32)
+    _mc_lib().ora_sample_rayleigh(n, C.c_uint64(seed), c.ctypes.data)
+    return c

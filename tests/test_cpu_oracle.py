@@ -184,3 +184,41 @@ def test_solid_angle_sums_to_rectangle():
     a, b = w * pix / sdd, h * pix / sdd
     omega = 4 * math.asin(math.sin(math.atan(a / 2)) * math.sin(math.atan(b / 2)))
     np.testing.assert_allclose(sa.astype(np.float64).sum(), omega, rtol=1e-4)
+
+
+def test_dense_reference_matches_transcription():
+    """The float64 grid_sample reference (used to check the adjoint) renders what
+    the transcription renders."""
+    import torch
+
+    from deepdrr_amd.autograd import PoseGeometry
+    from deepdrr_amd.material import attenuation_table
+    from deepdrr_amd.spectral_data import spectrum_tables, synthetic_120kv
+    from deepdrr_amd._abi_types import Scene
+    from tests.dense_ref import DenseDRR
+
+    rng = np.random.default_rng(0)
+    n = 14
+    dens = np.zeros((n, n, n), np.float32)
+    dens[2:-2, 2:-2, 2:-2] = rng.uniform(0.2, 1.8, size=(n - 4,) * 3)
+    labels = np.where(dens > 1.2, 1, np.where(dens > 0, 2, 0)).astype(np.uint8)
+    W = np.eye(4)
+    W[:3, :3] = [[1.0, 0, 0], [0, 0, 1.0], [0, -1.0, 0]]
+    W[:3, 3] = phantoms._centered_origin((n, n, n), (1, 1, 1))
+    geom = PoseGeometry((n, n, n), W)
+    projs = [phantoms.pose_projection(8, 8, 3.0, p, d, f)
+             for p, d, f in phantoms.batch_poses(2, seed=4, radius=2.0, cone_deg=40)]
+    theta = np.array([0.3, -0.2, 0.1, 0.02, -0.01, 0.03])
+    views = geom.views(projs, theta)
+    E, pdf = spectrum_tables(synthetic_120kv())
+    mu = attenuation_table(["air", "bone", "soft tissue"], E)
+    sc = Scene(step=0.25, n_materials=3, attenuate_outside=0, air_index=0)
+    d_kji = np.ascontiguousarray(np.moveaxis(dens, [0, 1, 2], [2, 1, 0]))
+    l_kji = np.ascontiguousarray(np.moveaxis(labels, [0, 1, 2], [2, 1, 0]))
+    ref = DenseDRR(views, W, geom.center, 0.25, E, pdf, mu, (n, n, n))
+    oh = torch.tensor(np.stack([(labels == m).astype(np.float64) for m in range(3)]))
+    I64 = ref.render(torch.tensor(dens.astype(np.float64)), oh,
+                     torch.tensor(theta, dtype=torch.float64)).numpy()
+    for i, (view, _) in enumerate(views):
+        I, _, _ = ctrans.project([(d_kji, l_kji)], view, sc, E, pdf, mu, 4)
+        np.testing.assert_allclose(I, I64[i], rtol=2e-5)  # f32 vs f64 geometry and integrand
