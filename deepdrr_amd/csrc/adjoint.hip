@@ -60,7 +60,7 @@ __device__ __forceinline__ void flush_cell(const VolDev &vol, float *gD, float b
 template <int M, bool POSE>
 __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, int udx, int vdx,
                                               float step, const float (&graw)[M], const AdjArgs &a,
-                                              float (&gpose)[6]) {
+                                              double (&gpose)[6]) {
     RayV1 r;
     march_setup_v1(g, udx, vdx, step, r);
     if (!r.trace) return;
@@ -99,7 +99,8 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
         }
     };
     if (c.have) set_h();
-    float S0[3] = {0.f, 0.f, 0.f}, S1[3] = {0.f, 0.f, 0.f};
+    // per-ray pose sums in FP64: they add thousands of terms of both signs
+    double S0[3] = {0.0, 0.0, 0.0}, S1[3] = {0.0, 0.0, 0.0};
     while (t < N && alpha <= r.mxv) {
         const float px = sx + alpha * rix - 1.0f;
         const float py = sy + alpha * riy - 1.0f;
@@ -136,8 +137,8 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const float gp = wt * (sw * gr[k] + rho * gh[k]);
-                S0[k] += gp;
-                S1[k] += alpha * gp;
+                S0[k] += (double)gp;
+                S1[k] += (double)alpha * (double)gp;
             }
         }
         alpha += step;
@@ -149,14 +150,14 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
         // dp/dt = -A, dp/dw = A [x - c]_x  ->  dL/dt = -A^T S0,
         // dL/dw = (A^T S0) x (s_w - c) + (A^T S1) x r_w
         const float *A = g.A;
-        float V0[3], V1[3];
+        double V0[3], V1[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            V0[j] = A[j] * S0[0] + A[4 + j] * S0[1] + A[8 + j] * S0[2];
-            V1[j] = A[j] * S1[0] + A[4 + j] * S1[1] + A[8 + j] * S1[2];
+            V0[j] = (double)A[j] * S0[0] + (double)A[4 + j] * S0[1] + (double)A[8 + j] * S0[2];
+            V1[j] = (double)A[j] * S1[0] + (double)A[4 + j] * S1[1] + (double)A[8 + j] * S1[2];
         }
-        const float q[3] = {a.sw[0] - a.c[0], a.sw[1] - a.c[1], a.sw[2] - a.c[2]};
-        const float rw[3] = {r.rx, r.ry, r.rz};
+        const double q[3] = {(double)a.sw[0] - a.c[0], (double)a.sw[1] - a.c[1], (double)a.sw[2] - a.c[2]};
+        const double rw[3] = {r.rx, r.ry, r.rz};
         gpose[0] -= V0[0];
         gpose[1] -= V0[1];
         gpose[2] -= V0[2];
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, Spectrum
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int udx, vdx;
     lane_pixel(bx * 2 + (wave & 1), by * 2 + (wave >> 1), lane, udx, vdx);
-    float gpose[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    double gpose[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (udx < g.width && vdx < g.height) {
         const int64_t idx = (int64_t)vdx * g.width + udx;
         const float gi = a.gI ? a.gI[idx] : 0.0f;
@@ -208,9 +209,9 @@ __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, Spectrum
     if (POSE) {
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-            float v = gpose[k];
+            double v = gpose[k];
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            if (lane == 0 && v != 0.0f) unsafeAtomicAdd(a.gPose + k, v);
+            if (lane == 0 && v != 0.0) unsafeAtomicAdd(a.gPose + k, (float)v);
         }
     }
 }

@@ -151,9 +151,12 @@ def test_adjoint_density_only_and_no_pose(gpu):
 
 
 def test_adjoint_fd_single_material_smooth(gpu):
-    """One material and a smooth density: the model is C1 up to small slope jumps,
-    so FP64 central FD at eps = 1e-3 agrees with the adjoint to 1e-4 for the
-    density (gradient direction) and all 6 pose DoF (relative L2)."""
+    """One material and a smooth density.  The density derivative is exact under
+    FD (the model is linear in density): 1e-4 at eps = 1e-3.  In the pose the
+    model is piecewise trilinear (slope jumps at cell faces), so the adjoint is
+    checked against the FP64 autograd of the same model to 1e-4 and FD is
+    required to converge to it: error at eps = 1e-5 below 1e-3 and below the
+    error at eps = 1e-3."""
     dens, labels, W, projs, step = _scene(n=20, n_views=3, det=12, seed=2)
     labels = np.where(dens > 0, 2, 0).astype(np.uint8)  # soft tissue inside, air border
     dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
@@ -180,13 +183,20 @@ def test_adjoint_fd_single_material_smooth(gpu):
         fd = (loss(d0 + eps * v, z) - loss(d0 - eps * v, z)) / (2 * eps)
         an = float((gD * v.numpy()).sum())
         assert abs(an - fd) <= 1e-4 * abs(fd)
-        fdp = np.zeros(6)
-        for k in range(6):
-            e = torch.zeros(6, dtype=torch.float64)
-            e[k] = eps
-            fdp[k] = (loss(d0, e) - loss(d0, -e)) / (2 * eps)
-        err = np.linalg.norm(gth - fdp) / np.linalg.norm(fdp)
-        print(f"single material: density FD {fd:.8g} vs {an:.8g}; pose FD {fdp} vs {gth}; rel {err:.2e}")
-        assert err < 1e-4
+        th64 = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+        (ref.render(d0, oh, th64) * gIt).sum().backward()
+        err_dense = np.linalg.norm(gth - th64.grad.numpy()) / np.linalg.norm(th64.grad.numpy())
+        assert err_dense < 1e-4
+        errs = []
+        for e_pose in (1e-3, 1e-5):
+            fdp = np.zeros(6)
+            for k in range(6):
+                e = torch.zeros(6, dtype=torch.float64)
+                e[k] = e_pose
+                fdp[k] = (loss(d0, e) - loss(d0, -e)) / (2 * e_pose)
+            errs.append(np.linalg.norm(gth - fdp) / np.linalg.norm(fdp))
+        print(f"single material: density FD {fd:.8g} vs {an:.8g}; pose vs dense {err_dense:.2e}; "
+              f"pose vs FD at 1e-3 / 1e-5: {errs[0]:.2e} / {errs[1]:.2e}")
+        assert errs[1] < 1e-3 and errs[1] < errs[0]
     finally:
         dp.close()

@@ -20,7 +20,14 @@ pytestmark = pytest.mark.gpu
 
 
 def _phantom(n=24):
-    hu = phantoms.clinical_ct_hu((n, n, n), seed=2)
+    """Tissue block with a lung-like cavity, a bone cylinder and a titanium rod."""
+    rng = np.random.default_rng(2)
+    hu = (40.0 + 30.0 * rng.standard_normal((n, n, n))).astype(np.float32)
+    i, j, k = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    hu[(i - n / 3) ** 2 + (j - n / 2) ** 2 < (n / 6) ** 2] = -800.0
+    hu[(i - 2 * n / 3) ** 2 + (j - n / 2) ** 2 < (n / 8) ** 2] = 1200.0
+    hu[(i - n / 2) ** 2 + (k - n / 2) ** 2 < 2.0] = 5900.0
+    hu[:, :, :1] = -1000.0
     vol = phantoms.Volume.from_hu(hu, origin=phantoms._centered_origin((n, n, n), (2, 2, 2)), spacing=(2, 2, 2),
                                   anatomical_coordinate_system="LPS",
                                   material_classes=phantoms.FIVE_MATERIAL_CLASSES)
@@ -61,8 +68,9 @@ def test_mc_matches_cpu(gpu):
             frac = float(np.mean(np.abs(z) <= 3.0))
             print(f"{name}: {sel.sum()} pixels, within 3 sigma {frac:.4f}")
             assert frac >= 0.99, name
-        # scatter is present and a plausible share of the signal
-        assert 0.01 < g[1:].sum() / g.sum() < 0.9
+        # scatter is present (the 128 mm detector subtends little of the 4 pi
+        # the 48 mm object scatters into, so its share is well under 10%)
+        assert 0.002 < g[1:].sum() / g.sum() < 0.5
         _abi.lib().drr_volume_destroy(h)
     finally:
         _abi.lib().drr_ctx_destroy(ctx)
