@@ -1,19 +1,25 @@
 """DRR benchmark on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--mode batched|plain] [--batch B]
+    python bench.py [--steps K] [--warmup W] [--batch B] [--quick]
 
-Workload: 512x512x400 int16 HU phantom (0.8 x 0.8 x 1 mm, five materials)
-projected onto a 1536x1536 detector at 0.194 mm, SDD 1020 mm, step 0.1 mm,
-20-bin 120 kV spectrum, neglog; synthetic seeded data.
+Headline workload (the metric's): a 512x512x400 int16 HU phantom (0.8 x 0.8 x
+1 mm, five materials) projected onto a 1536x1536 detector at 0.194 mm, SDD
+1020 mm, step 0.1 mm, 20-bin 120 kV spectrum, neglog; synthetic seeded data.
 
-A step is one pass of the hot path over one batch of views:
-  * batched: B poses (sampled in a 100 mm ball / 30 degree cone) in one
-    persistent launch replayed from a hipGraph, device neglog, one D2H copy;
-  * plain: the reference's flow per view -- host geometry, one launch, sync,
-    D2H of intensity and photon_prob, host NumPy neglog.
-Prints one JSON line (metric views/s).  Extra fields: the plain path's
-views/s, the roofline of the dominant kernel (HIP events on its stream) and
-the CPU transcription's throughput on a bounded sample of the same view.
+A step is one pass of the hot path over one batch of B views: B poses (100 mm
+ball / 30 degree cone / source fraction U(0.5, 0.9)) in one persistent launch
+replayed from a hipGraph, device neglog, one device-to-host copy of the
+images.  `value` is batched views/s.  Also reported (one JSON line):
+  * plain_views_per_s -- the reference's flow per view (host geometry, one
+    launch, sync, D2H of intensity and photon_prob, NumPy neglog), same AP view;
+  * mixed_batch -- 256 poses over 3 volumes of uneven size and detectors of
+    384^2 / 768^2 / 1536^2 in one persistent launch;
+  * adjoint -- 192^3 density, 64 views of 256^2: backward ms/view and its
+    ratio to the forward;
+  * monte_carlo -- Woodcock transport of photon histories into the 512x512x400
+    phantom onto a 256^2 detector: histories/s and flight steps/s;
+  * roofline of the dominant kernel (HIP events on its stream) and the CPU
+    transcription's throughput on a bounded sample of the same view.
 """
 from __future__ import annotations
 
@@ -30,33 +36,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # non-packed FP32 lane-ops/s, 256 CUs @ 2.4 GHz
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_scene(_abi, ctx, materials, step):
+def make_ctx(_abi, materials, step):
     from deepdrr_amd._abi_types import Scene
     from deepdrr_amd.material import attenuation_table
     from deepdrr_amd.spectral_data import spectrum_tables, synthetic_120kv
 
+    ctx = C.c_void_p()
+    _abi.call("drr_ctx_create", 0, C.byref(ctx))
     sc = Scene(step=step, n_materials=len(materials), attenuate_outside=0, air_index=0)
     _abi.call("drr_ctx_set_scene", ctx, C.byref(sc))
     E, pdf = spectrum_tables(synthetic_120kv())
     mu = attenuation_table(materials, E)
     _abi.call("drr_ctx_set_spectrum", ctx, len(E), E.ctypes.data, pdf.ctypes.data, mu.ctypes.data)
-    return E, pdf, mu
+    return ctx, (E, pdf, mu, sc)
 
 
 def plain_views(torch, _abi, ctx, varr, views, dev, utils):
-    """The reference's per-view flow; returns the images [N,H,W] and kernel ms per view."""
-    imgs = []
-    kms = []
+    """The reference's per-view flow; returns images [N,H,W] and kernel ms per view."""
+    imgs, kms = [], []
     for view in views:
-        H, W = view.height, view.width
-        I = torch.empty((H, W), dtype=torch.float32, device=dev)
+        I = torch.empty((view.height, view.width), dtype=torch.float32, device=dev)
         P = torch.empty_like(I)
         _abi.call("drr_project", ctx, varr, C.byref(view), _abi.ptr(I), _abi.ptr(P), None, None,
                   _abi.stream_ptr())
@@ -70,7 +75,7 @@ def plain_views(torch, _abi, ctx, varr, views, dev, utils):
 
 
 def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=15.0):
-    """Oracle transcription on the host cores over a band of detector rows."""
+    """Oracle transcription on the host cores over a band of detector rows of the view."""
     from oracle import ctrans
 
     from deepdrr_amd._abi_types import View
@@ -82,7 +87,6 @@ def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=15.0):
         C.pointer(sub)[0] = view
         W = np.array(view.world_from_index[:], np.float32)
         v0 = H // 2 - rows // 2
-        # shift the pixel origin to row v0 (u*W0 + (v+v0)*W1 + W2)
         W2 = (np.float32(v0) * W[[1, 4, 7]] + W[[2, 5, 8]]).astype(np.float32)
         sub.world_from_index[2], sub.world_from_index[5], sub.world_from_index[8] = map(float, W2)
         sub.height = rows
@@ -94,8 +98,119 @@ def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=15.0):
         rows = min(H, int(rows * max(2.0, budget_s / 4 / max(dt, 1e-3))))
     frac = rows / H
     return dict(value=frac / dt, unit="views/s", cores=threads, kind="port",
-                sample=f"{rows} of {H} detector rows of the AP view ({frac:.4f} of the view), "
+                sample=f"{rows} of {H} detector rows of the AP 1536x1536 view ({frac:.4f} of the view), "
                        f"{dt:.1f} s, oracle/transcription.c with OpenMP")
+
+
+def measure_mixed(torch, _abi, ctx, clinical_handle, geom_clinical, step, B=256):
+    """256 poses over 3 volumes (512x512x400, 256x256x300, 128^3) and detectors
+    384^2 @ 0.776, 768^2 @ 0.388, 1536^2 @ 0.194 mm in one persistent launch."""
+    from deepdrr_amd import phantoms, workload
+    from deepdrr_amd import projector as P
+    from deepdrr_amd.batch import BatchPlan
+
+    mats = sorted(workload.CLINICAL_CLASSES)
+    vol2 = phantoms.clinical_ct((256, 256, 300), spacing=(1.0, 1.0, 1.0), seed=1)
+    vol3 = phantoms.small_ct(128, seed=2)
+    handles = [clinical_handle]
+    for v in (vol2, vol3):
+        d, l = P.volume_arrays(v, mats)
+        h = C.c_void_p()
+        nz, ny, nx = d.shape
+        _abi.call("drr_volume_create", ctx, d.ctypes.data, l.ctypes.data, nx, ny, nz, 0, C.byref(h))
+        handles.append(h)
+    geoms = [geom_clinical, vol2, vol3]
+    dets = [(384, 0.776), (768, 0.388), (1536, 0.194)]
+    views, vidx = [], []
+    for i, (p, d, f) in enumerate(phantoms.batch_poses(B, seed=0)):
+        vi = i % 3
+        n, pix = dets[(i // 3) % 3]
+        c = np.array(geoms[vi].center_in_world)
+        proj = phantoms.pose_projection(n, n, pix, p, d, f, center=c)
+        views.append(P.build_view([geoms[vi]], [0], proj, 4 * 1020.0))
+        vidx.append(vi)
+    samples = sum(workload.count_samples(v, step, stride=4) for v in views)
+    plan = BatchPlan(ctx, handles, views, vidx, neglog=True)
+    plan.launch()
+    plan.fetch("images")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plan.launch()
+    plan.fetch("images")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    ms = plan.march_ms()
+    plan.close()
+    for h in handles[1:]:
+        _abi.lib().drr_volume_destroy(h)
+    return dict(views=B, views_per_s=B / dt, seconds=dt, march_ms=ms, samples=samples,
+                samples_per_s=samples / (ms * 1e-3),
+                detectors="384^2/768^2/1536^2", volumes="512x512x400, 256x256x300, 128^3")
+
+
+def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1):
+    """Forward and backward (density + pose) of 64 views 256^2 of a 192^3 volume."""
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.autograd import DifferentiableProjector
+    from deepdrr_amd.spectral_data import synthetic_120kv
+
+    hu = phantoms.small_ct_hu(n, seed=0)
+    vol = phantoms.Volume.from_hu(hu, origin=phantoms._centered_origin((n,) * 3, (1, 1, 1)),
+                                  anatomical_coordinate_system="LPS")
+    mats = sorted(vol.materials[0].keys())
+    remap = np.array([mats.index(k) for k in vol.materials[0]], np.uint8)
+    labels = remap[vol.materials[1]]
+    dp = DifferentiableProjector(labels, mats, vol.world_from_IJK.data, step=step, spectrum=synthetic_120kv())
+    ang = np.linspace(0, 2 * np.pi, n_views, endpoint=False)
+    projs = [phantoms.pose_projection(det, det, 1.6, [0, 0, 0], [np.sin(a), np.cos(a), 0.0], 0.5)
+             for a in ang]
+    dens = torch.tensor(vol.data, device="cuda", requires_grad=True)
+    theta = torch.zeros(6, device="cuda", requires_grad=True)
+    target = None
+    for it in range(2):  # warm-up, then timed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        I = dp(dens, theta, projs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if target is None:
+            target = (I.detach() * 0.98)
+        loss = ((I - target) ** 2).mean()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        dens.grad = None
+        theta.grad = None
+    dp.close()
+    fwd = (t1 - t0) * 1e3 / n_views
+    bwd = (t3 - t2) * 1e3 / n_views
+    return dict(volume=f"{n}^3", views=n_views, detector=f"{det}^2", forward_ms_per_view=fwd,
+                adjoint_ms_per_view=bwd, adjoint_over_forward=bwd / fwd,
+                note="wall time of one drr_project_jobs / one drr_adjoint_jobs launch over all views "
+                     "(density and 6-DoF pose gradients)")
+
+
+def measure_mc(torch, _abi, ctx, handle, geom, n_hist):
+    from deepdrr_amd import mc, phantoms, workload
+    from deepdrr_amd.spectral_data import synthetic_120kv
+
+    mats = sorted(workload.CLINICAL_CLASSES)
+    dev = phantoms.ap_device(256, 256, 1.164, sdd=1020.0)
+    p = mc.make_params(mats, synthetic_120kv(), dev.get_camera_projection(), 1020.0,
+                       geom.world_from_IJK.data)
+    run = mc.MonteCarlo(ctx, handle, p)
+    run.run(min(n_hist, 1 << 20), seed=1)  # warm-up (also builds the voxel-order copy)
+    torch.cuda.synchronize()
+    run = mc.MonteCarlo(ctx, handle, p)
+    t = time.perf_counter()
+    run.run(n_hist, seed=12345)
+    img, _, steps = run.results()
+    dt = time.perf_counter() - t
+    return dict(histories=n_hist, seconds=dt, histories_per_s=n_hist / dt, flight_steps=steps,
+                flight_steps_per_s=steps / dt, gather_GB_per_s=5.0 * steps / dt / 1e9,
+                scatter_fraction=float(img[1:].sum() / img.sum()), detector="256^2 at 1.164 mm")
 
 
 def main():
@@ -103,13 +218,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--tuning", type=int, nargs=4, default=[8, 8, 4, 8],
                     help="tile_w tile_h waves_per_block blocks_per_cu")
-    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--step-mm", type=float, default=0.1)
-    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-views", type=int, default=3)
+    ap.add_argument("--mc-histories", type=int, default=100_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline and plain path only")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint measurement (192^3, 64 views of 256^2)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprof)")
     args = ap.parse_args()
@@ -123,6 +241,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
         return
+    if args.adjoint_only:
+        import torch
+
+        print(json.dumps({"adjoint": measure_adjoint(torch)}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -132,14 +255,12 @@ def main():
 
     from deepdrr_amd import _abi, phantoms, utils, workload
     from deepdrr_amd import projector as P
-    from deepdrr_amd._abi_types import Scene
+    from deepdrr_amd.batch import BatchPlan
 
     dev = torch.device("cuda", 0)
     torch.cuda.init()
     materials = sorted(workload.CLINICAL_CLASSES)
-    ctx = C.c_void_p()
-    _abi.call("drr_ctx_create", 0, C.byref(ctx))
-    E, pdf, mu = make_scene(_abi, ctx, materials, args.step_mm)
+    ctx, (E, pdf, mu, scene) = make_ctx(_abi, materials, args.step_mm)
     log("generating the 512x512x400 phantom")
     t = time.perf_counter()
     vh, hu = workload.clinical_volume_device(ctx, materials)
@@ -147,100 +268,83 @@ def main():
     log(f"phantom + GPU ingestion {time.perf_counter() - t:.1f} s")
     varr = (C.c_void_p * 1)(vh.value)
     geom = workload.clinical_geometry_volume()
-    ap_dev = phantoms.ap_device(1536, 1536, 0.194)
-    ap_view = P.build_view([geom], [0], ap_dev.get_camera_projection(), 4 * 1020.0)
-    n_samples_ap = workload.count_samples(ap_view, args.step_mm, stride=2)
+    ap_view = P.build_view([geom], [0], phantoms.ap_device(1536, 1536, 0.194).get_camera_projection(),
+                           4 * 1020.0)
 
-    # ---- plain path (reference flow), always measured -----------------------
-    plain_n = args.plain_views
-    log("plain warmup")
+    # ---- plain path: the reference's flow, one view per launch ----------------
     plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    _, kms_plain = plain_views(torch, _abi, ctx, varr, [ap_view] * plain_n, dev, utils)
+    _, kms_plain = plain_views(torch, _abi, ctx, varr, [ap_view] * args.plain_views, dev, utils)
     torch.cuda.synchronize()
-    plain_vps = plain_n / (time.perf_counter() - t)
+    plain_vps = args.plain_views / (time.perf_counter() - t)
     plain_kernel_ms = float(np.mean(kms_plain))
-    log(f"plain: {plain_vps:.3f} views/s, march kernel {plain_kernel_ms:.2f} ms/view")
+    n_samples_ap = workload.count_samples(ap_view, args.step_mm, stride=2)
+    log(f"plain: {plain_vps:.2f} views/s, march {plain_kernel_ms:.2f} ms/view")
 
-    result = dict(metric="views/s (512x512x400 CT->1536^2 det) batched vs plain; adjoint ms/view",
-                  unit="views/s", n_gpus=1, higher_is_better=True, scaling="weak", vs_baseline=None,
-                  dtype="f32", data="synthetic")
-    steps, warm = args.steps, args.warmup
-    if args.mode == "plain":
-        for _ in range(warm):
-            plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        kms = []
-        for _ in range(steps):
-            _, k = plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
-            kms += k
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        value = steps / dt
-        kernel_ms = float(np.mean(kms))
-        samples_per_launch = n_samples_ap
-        kernel_name = "k_project_v1<5,false>"
-        config = {"workload": "plain forward, 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
-                              "1536x1536 detector at 0.194 mm, AP view, step 0.1 mm, 20-bin 120 kV, neglog",
-                  "views_per_step": 1, "mode": "plain"}
-    else:
-        from deepdrr_amd.batch import BatchPlan
-
-        B = args.batch
-        poses = phantoms.batch_poses(B, seed=0)
-        views = [P.build_view([geom], [0], phantoms.pose_projection(1536, 1536, 0.194, p, d, f),
-                              4 * 1020.0) for p, d, f in poses]
-        log("counting in-volume samples of the batch")
-        samples_per_launch = sum(workload.count_samples(v, args.step_mm, stride=2) for v in views)
-        plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True, tuning=tuple(args.tuning))
-        log(f"batched warmup ({B} views per launch)")
-        for _ in range(warm):
-            plan.launch()
-            plan.fetch("images")
-        torch.cuda.synchronize()
-        kms = []
-        t = time.perf_counter()
-        for _ in range(steps):
-            plan.launch()
-            plan.fetch("images")       # one D2H of the B neglog images
-            kms.append(plan.march_ms())
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        value = B * steps / dt
-        kernel_ms = float(np.mean(kms))
-        kernel_name = "k_batch_march<5,false,8>"
-        plan.close()
-        config = {"workload": f"batched forward, {B} poses (100 mm ball, 30 deg cone, source fraction "
-                              "U(0.5,0.9)) of a 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
-                              "1536x1536 detector at 0.194 mm, step 0.1 mm, 20-bin 120 kV, device neglog, "
-                              "one persistent launch per step replayed from a hipGraph",
-                  "views_per_step": B, "mode": "batched", "tuning": list(args.tuning)}
-    ms_per_step = dt / steps * 1e3
+    # ---- headline: batched clinical views ---------------------------------------
+    B = args.batch
+    views = [P.build_view([geom], [0], phantoms.pose_projection(1536, 1536, 0.194, p, d, f), 4 * 1020.0)
+             for p, d, f in phantoms.batch_poses(B, seed=0)]
+    samples_per_launch = sum(workload.count_samples(v, args.step_mm, stride=2) for v in views)
+    plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True, tuning=tuple(args.tuning))
+    for _ in range(args.warmup):
+        plan.launch()
+        plan.fetch("images")
+    torch.cuda.synchronize()
+    kms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        plan.launch()
+        plan.fetch("images")
+        kms.append(plan.march_ms())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    plan.close()
+    value = B * args.steps / dt
+    kernel_ms = float(np.mean(kms))
     achieved = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
-    result.update(value=value, steps=args.steps, warmup=args.warmup, ms_per_step=ms_per_step,
-                  config=config,
-                  plain_views_per_s=plain_vps, plain_kernel_ms=plain_kernel_ms,
-                  batched_over_plain=value / plain_vps,
-                  roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                            "kernel": kernel_name, "kernel_ms": kernel_ms,
-                            "samples_per_launch": samples_per_launch,
-                            "samples_per_s": samples_per_launch / (kernel_ms * 1e-3)})
+    log(f"batched: {value:.2f} views/s, march {kernel_ms / B:.2f} ms/view")
+
+    result = dict(
+        metric="views/s (512x512x400 CT->1536^2 det) batched vs plain; adjoint ms/view",
+        value=value, unit="views/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=dt / args.steps * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32", data="synthetic",
+        config={"workload": f"batched forward: {B} poses (100 mm ball, 30 deg cone, source fraction "
+                            "U(0.5,0.9)) of a 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
+                            "1536x1536 detector at 0.194 mm, SDD 1020 mm, step 0.1 mm, 20-bin 120 kV, "
+                            "device neglog, one persistent launch per step replayed from a hipGraph",
+                "views_per_step": B, "tuning": list(args.tuning)},
+        plain_views_per_s=plain_vps, plain_march_ms_per_view=plain_kernel_ms,
+        batched_march_ms_per_view=kernel_ms / B, batched_over_plain=value / plain_vps,
+        roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_march<5,false,8>",
+                  "kernel_ms": kernel_ms, "samples_per_launch": samples_per_launch,
+                  "bytes_per_sample": workload.BYTES_PER_SAMPLE,
+                  "samples_per_s": samples_per_launch / (kernel_ms * 1e-3)},
+    )
+    if not args.quick:
+        log("mixed batch (3 volumes, 3 detector sizes, 256 poses)")
+        result["mixed_batch"] = measure_mixed(torch, _abi, ctx, vh, geom, args.step_mm)
+        log(f"mixed: {result['mixed_batch']['views_per_s']:.1f} views/s")
+        log("Monte Carlo")
+        result["monte_carlo"] = measure_mc(torch, _abi, ctx, vh, geom, args.mc_histories)
+        log(f"mc: {result['monte_carlo']['histories_per_s']:.3g} histories/s")
     if not args.no_cpu:
         log("cpu baseline")
-        from deepdrr_amd._abi_types import Scene as S
-        hu_kji = np.ascontiguousarray(np.moveaxis(hu, [0, 1, 2], [2, 1, 0]))
         del hu
-        dens = np.empty(hu_kji.shape, np.float32)
-        labs = np.empty(hu_kji.shape, np.uint8)
+        dens = np.empty((400, 512, 512), np.float32)
+        labs = np.empty((400, 512, 512), np.uint8)
         _abi.call("drr_volume_download", vh, dens.ctypes.data, labs.ctypes.data)
-        del hu_kji
-        sc = S(step=args.step_mm, n_materials=len(materials), attenuate_outside=0, air_index=0)
-        result["cpu_baseline"] = cpu_baseline(ap_view, [(dens, labs)], sc, E, pdf, mu)
+        result["cpu_baseline"] = cpu_baseline(ap_view, [(dens, labs)], scene, E, pdf, mu)
+        del dens, labs
     _abi.lib().drr_volume_destroy(vh)
     _abi.lib().drr_ctx_destroy(ctx)
+    if not args.quick:
+        log("adjoint (192^3, 64 views of 256^2)")
+        result["adjoint"] = measure_adjoint(torch)
+        log(f"adjoint: {result['adjoint']['adjoint_ms_per_view']:.2f} ms/view")
     print(json.dumps(result), flush=True)
 
 

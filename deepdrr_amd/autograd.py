@@ -2,8 +2,9 @@
 
 deepdrr is not a differentiable renderer (README.md:47).  This module adds a
 ``torch.autograd.Function`` whose forward is the engine's single-volume
-projector (drr_project, the semantics of project_kernel.cu) and whose backward
-is the hand-written adjoint kernel (drr_adjoint): the per-ray march is
+projector (drr_project_jobs: every view in one persistent launch, the
+semantics of project_kernel.cu) and whose backward is the hand-written adjoint
+kernel (drr_adjoint_jobs, likewise one launch): the per-ray march is
 recomputed, density gradients are scattered atomically with the trilinear
 weights, and pose gradients come from the sample-position derivatives.
 
@@ -22,7 +23,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _abi, geo
-from ._abi_types import Scene, make_view
+from ._abi_types import Job, Scene, make_view
 from .material import attenuation_table
 from .projector import world_from_index_rows
 from .spectral_data import get_spectrum, spectrum_tables
@@ -144,30 +145,33 @@ def _make_function():
             dens_kji = density.detach().permute(2, 1, 0).contiguous().float()
             th = theta.detach().double().cpu().numpy()
             views = dp.views(projections, th)
+            H, W = views[0][0].height, views[0][0].width
+            if any(v.height != H or v.width != W for v, _ in views):
+                raise ValueError("DifferentiableProjector: all projections need the same sensor size")
+            jobs = (Job * len(views))()
+            for i, (view, _) in enumerate(views):
+                jobs[i].view = view
+                jobs[i].volume_index = 0
+                jobs[i].out_offset = i * H * W
             h = dp.upload(dens_kji)
             varr = (C.c_void_p * 1)(h.value)
-            outs = []
+            I = torch.empty((len(views), H, W), dtype=torch.float32, device=dp.device)
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
             try:
-                for view, _ in views:
-                    I = torch.empty((view.height, view.width), dtype=torch.float32, device=dp.device)
-                    P = torch.empty_like(I)
-                    _abi.call("drr_project", dp.ctx, varr, C.byref(view), _abi.ptr(I), _abi.ptr(P),
-                              None, None, stream)
-                    outs.append(I)
-                torch.cuda.synchronize(dp.device)
+                _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I), None, stream)
             except Exception:
                 _abi.lib().drr_volume_destroy(h)
                 raise
             ctx.dp = dp
-            ctx.views = views
+            ctx.jobs = jobs
+            ctx.sources = np.ascontiguousarray(np.stack([src for _, src in views]), np.float32)
             ctx.vol = h
             if not (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]):
                 _abi.lib().drr_volume_destroy(h)
                 ctx.vol = None
             ctx.shape = density.shape
             ctx.need_pose = theta.requires_grad
-            return torch.stack(outs)
+            return I
 
         @staticmethod
         def backward(ctx, grad_out):
@@ -179,11 +183,8 @@ def _make_function():
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
             c = np.asarray(dp.center, np.float32)
             try:
-                for i, (view, src) in enumerate(ctx.views):
-                    g = grad_out[i]
-                    _abi.call("drr_adjoint", dp.ctx, ctx.vol, C.byref(view), _abi.ptr(g), None,
-                              src.ctypes.data, c.ctypes.data, _abi.ptr(gD), _abi.ptr(gP), stream)
-                torch.cuda.synchronize(dp.device)
+                _abi.call("drr_adjoint_jobs", dp.ctx, ctx.vol, ctx.jobs, len(ctx.jobs), _abi.ptr(grad_out),
+                          None, ctx.sources.ctypes.data, c.ctypes.data, _abi.ptr(gD), _abi.ptr(gP), stream)
             finally:
                 _abi.lib().drr_volume_destroy(ctx.vol)
                 ctx.vol = None

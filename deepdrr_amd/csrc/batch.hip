@@ -1,50 +1,121 @@
 // batch.hip -- batched forward projection: hundreds of (volume, pose,
-// detector) jobs in one persistent launch, replayed from a hipGraph.
+// detector) jobs in one persistent launch, replayed from a hipGraph
+// (drr_batch_*) or launched once (drr_project_jobs).
 //
-// This is the added batched form of deepdrr's per-view loop
-// (deepdrr/projector/projector.py:679-685 launching projectKernel once per
-// view).  Each job is one view of one volume; its detector is cut into ray
-// tiles of 64 pixels (one wavefront each).  The tile list is grouped by
-// volume and job and split into 8 contiguous segments, one per XCD: a
-// persistent wave pulls tiles from its own XCD's segment (atomic counter) and
-// steals from the others once it is empty, so each XCD's L2 works on a
-// coherent range of poses/rays and the tail is balanced at tile granularity.
-// The per-ray march is the same device code as drr_project, so every image is
-// bit-identical to the one-view path.
+// This is the batched form of deepdrr's per-view loop
+// (deepdrr/projector/projector.py:679-685, one projectKernel launch per view).
+// Scheduling is described in jobs.h.  The per-ray march is the same device
+// code as drr_project, so every image is bit-identical to the one-view path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <numeric>
 #include <vector>
 
 #include "drr_internal.h"
 #include "epilogue.h"
+#include "jobs.h"
 #include "march.h"
 
 using namespace drr;
 
-struct JobDev {
-    Geo1 g;
-    int vol;
-    int tiles_x;
-    int64_t out_offset;
-};
+__global__ void k_jobs_reset(int *counters) {
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+}
+
+void drr_jobs_free(JobTable &t) {
+    (void)hipFree(t.d_jobs);
+    (void)hipFree(t.d_vols);
+    (void)hipFree(t.d_items);
+    (void)hipFree(t.d_seg);
+    (void)hipFree(t.d_counters);
+    t.d_jobs = nullptr;
+    t.d_vols = nullptr;
+    t.d_items = nullptr;
+    t.d_seg = nullptr;
+    t.d_counters = nullptr;
+}
+
+int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                   int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
+    if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
+        return drr_set_error(DRR_ERR_INVALID, "jobs: need >= 1 volume and 1..65535 jobs");
+    if (!((tune.tile_w == 8 && tune.tile_h == 8) || (tune.tile_w == 16 && tune.tile_h == 4)))
+        return drr_set_error(DRR_ERR_INVALID, "jobs: tile must be 8x8 or 16x4");
+    if (tune.waves_per_block < 1 || tune.waves_per_block > 4 || tune.blocks_per_cu < 1 || tune.blocks_per_cu > 32)
+        return drr_set_error(DRR_ERR_INVALID, "jobs: waves_per_block in [1,4], blocks_per_cu in [1,32]");
+    for (int v = 0; v < n_vols; v++)
+        if (!vols[v] || vols[v]->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "jobs: NULL or foreign volume");
+    t.n_jobs = n_jobs;
+    t.n_vols = n_vols;
+    std::vector<JobDev> hj(n_jobs);
+    t.imgs.assign(n_jobs, ImgSeg{0, 0});
+    std::vector<int> order(n_jobs);
+    t.max_npix = 0;
+    for (int j = 0; j < n_jobs; j++) {
+        const drr_job_t &J = jobs[j];
+        if (J.view.n_volumes != 1)
+            return drr_set_error(DRR_ERR_UNSUPPORTED, "jobs: each job must view exactly one volume");
+        if (J.volume_index < 0 || J.volume_index >= n_vols)
+            return drr_set_error(DRR_ERR_INVALID, "jobs: volume_index out of range");
+        if (J.view.width < 1 || J.view.height < 1 || J.view.width > 65536 || J.view.height > 65536 || J.out_offset < 0)
+            return drr_set_error(DRR_ERR_INVALID, "jobs: bad image size or offset");
+        if (drr_check_view_finite(&J.view) != DRR_OK) return DRR_ERR_INVALID;
+        view_to_geo1(J.view, hj[j].g);
+        hj[j].vol = J.volume_index;
+        hj[j].tiles_x = (J.view.width + tune.tile_w - 1) / tune.tile_w;
+        hj[j].out_offset = J.out_offset;
+        t.imgs[j] = ImgSeg{J.out_offset, (int64_t)J.view.width * J.view.height};
+        t.max_npix = std::max<int64_t>(t.max_npix, t.imgs[j].npix);
+        order[j] = j;
+    }
+    // group tiles by volume (L2 / MALL reuse across poses), then by job
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hj[a].vol < hj[b].vol; });
+    std::vector<int2> items;
+    for (int j : order) {
+        const int ty = (jobs[j].view.height + tune.tile_h - 1) / tune.tile_h;
+        const int n = hj[j].tiles_x * ty;
+        for (int k = 0; k < n; k++) items.push_back(make_int2(j, k));
+    }
+    t.n_items = (int64_t)items.size();
+    if (t.n_items >= ((int64_t)1 << 31)) return drr_set_error(DRR_ERR_INVALID, "jobs: too many ray tiles");
+    int seg[9];
+    for (int q = 0; q <= 8; q++) seg[q] = (int)(t.n_items * q / 8);
+    std::vector<VolDev> hv(n_vols);
+    for (int v = 0; v < n_vols; v++) hv[v] = vol_to_dev(vols[v]);
+    hipError_t e;
+#define JCHK(x)                                             \
+    do {                                                    \
+        e = (x);                                            \
+        if (e != hipSuccess) {                              \
+            drr_jobs_free(t);                               \
+            return drr_hip_error(e, #x);                    \
+        }                                                   \
+    } while (0)
+    JCHK(hipMalloc(&t.d_jobs, sizeof(JobDev) * n_jobs));
+    JCHK(hipMalloc(&t.d_vols, sizeof(VolDev) * n_vols));
+    JCHK(hipMalloc(&t.d_items, sizeof(int2) * items.size()));
+    JCHK(hipMalloc(&t.d_seg, sizeof(int) * 9));
+    JCHK(hipMalloc(&t.d_counters, sizeof(int) * 8));
+    JCHK(hipMemcpy(t.d_jobs, hj.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_vols, hv.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_seg, seg, sizeof(int) * 9, hipMemcpyHostToDevice));
+    int cus = 0;
+    JCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+#undef JCHK
+    t.block = 64 * tune.waves_per_block;
+    t.grid = cus * tune.blocks_per_cu;
+    return DRR_OK;
+}
 
 struct drr_batch_s {
     drr_ctx *ctx = nullptr;
-    int n_jobs = 0, n_vols = 0;
     int M = 0;
     bool atten = false;
     drr_batch_tuning_t tune{};
-    JobDev *d_jobs = nullptr;
-    VolDev *d_vols = nullptr;
-    int2 *d_items = nullptr;
-    int *d_seg = nullptr;        // 9 segment starts
-    int *d_counters = nullptr;   // 8 queue counters
+    JobTable t;
     ImgSeg *d_imgs = nullptr;
     unsigned int *d_keys = nullptr;
-    int64_t n_items = 0, max_npix = 0;
-    int grid = 0, block = 0;
     hipStream_t cap = nullptr;
     hipGraph_t graph_march = nullptr, graph_epi = nullptr;
     hipGraphExec_t exec_march = nullptr, exec_epi = nullptr;
@@ -52,70 +123,44 @@ struct drr_batch_s {
     bool launched = false;
 };
 
-__global__ void k_batch_reset(int *counters) {
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
-}
-
 template <int M, bool ATTEN, int TW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M + (ATTEN ? 1 : 0)>::waves))) void k_batch_march(const JobDev *__restrict__ jobs,
-                                                     const VolDev *__restrict__ vols,
-                                                     const int2 *__restrict__ items,
-                                                     const int *__restrict__ seg, int *counters,
-                                                     SpectrumDev sp, float step, int air,
-                                                     float *__restrict__ I, float *__restrict__ P) {
-    constexpr int TH = 64 / TW;
-    const int lane = threadIdx.x & 63;
-    int q = blockIdx.x & 7;  // blocks are dispatched round-robin over the 8 XCDs
-    int exhausted = 0;
-    while (exhausted < 8) {
-        int pos = 0;
-        if (lane == 0) pos = atomicAdd(&counters[q], 1);
-        pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0));
-        const int idx = __builtin_amdgcn_readfirstlane(seg[q] + pos);
-        if (idx >= seg[q + 1]) {
-            q = (q + 1) & 7;
-            ++exhausted;
-            continue;
-        }
-        const int2 it = items[idx];
-        const int j = __builtin_amdgcn_readfirstlane(it.x);
-        const int tile = __builtin_amdgcn_readfirstlane(it.y);
-        const JobDev &J = jobs[j];
-        const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
-        const int udx = tx * TW + (lane % TW), vdx = ty * TH + (lane / TW);
-        if (udx < J.g.width && vdx < J.g.height) {
-            float ad[M];
-            march_v1<M, ATTEN>(J.g, vols[J.vol], udx, vdx, step, air, ad);
-            float inten, pp;
-            spectral<M>(sp, ad, inten, pp);
-            const int64_t o = J.out_offset + (int64_t)vdx * J.g.width + udx;
-            I[o] = inten;
-            P[o] = pp;
-        }
-    }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M + (ATTEN ? 1 : 0)>::waves)))
+void k_batch_march(const JobDev *__restrict__ jobs, const VolDev *__restrict__ vols,
+                   const int2 *__restrict__ items, const int *__restrict__ seg, int *counters,
+                   SpectrumDev sp, float step, int air, float *__restrict__ I, float *__restrict__ P) {
+    for_each_tile<TW>(jobs, items, seg, counters, [&](int, const JobDev &J, int udx, int vdx, bool valid) {
+        if (!valid) return;
+        float ad[M];
+        march_v1<M, ATTEN>(J.g, vols[J.vol], udx, vdx, step, air, ad);
+        float inten, pp;
+        spectral<M>(sp, ad, inten, pp);
+        const int64_t o = J.out_offset + (int64_t)vdx * J.g.width + udx;
+        I[o] = inten;
+        if (P) P[o] = pp;
+    });
 }
 
 template <int M, bool ATTEN>
-static void launch_march_tw(int tw, dim3 g, dim3 b, hipStream_t st, drr_batch *B, SpectrumDev sp,
-                            float step, int air, float *I, float *P) {
+static void launch_march_tw(int tw, const JobTable &t, hipStream_t st, SpectrumDev sp, float step, int air,
+                            float *I, float *P) {
+    dim3 g(t.grid), b(t.block);
     if (tw == 16)
-        k_batch_march<M, ATTEN, 16><<<g, b, 0, st>>>(B->d_jobs, B->d_vols, B->d_items, B->d_seg,
-                                                      B->d_counters, sp, step, air, I, P);
+        k_batch_march<M, ATTEN, 16><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
+                                                      step, air, I, P);
     else
-        k_batch_march<M, ATTEN, 8><<<g, b, 0, st>>>(B->d_jobs, B->d_vols, B->d_items, B->d_seg,
-                                                     B->d_counters, sp, step, air, I, P);
+        k_batch_march<M, ATTEN, 8><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
+                                                     step, air, I, P);
 }
 
-static void launch_march(drr_batch *B, hipStream_t st, SpectrumDev sp, float step, int air, float *I,
-                         float *P) {
-    dim3 g(B->grid), b(B->block);
-    switch (B->M) {
-#define DRR_CASE(MM)                                                                          \
-    case MM:                                                                                  \
-        if (B->atten)                                                                         \
-            launch_march_tw<MM, true>(B->tune.tile_w, g, b, st, B, sp, step, air, I, P);     \
-        else                                                                                  \
-            launch_march_tw<MM, false>(B->tune.tile_w, g, b, st, B, sp, step, air, I, P);    \
+static void launch_march(const JobTable &t, int M, bool atten, int tw, hipStream_t st, SpectrumDev sp,
+                         float step, int air, float *I, float *P) {
+    switch (M) {
+#define DRR_CASE(MM)                                                             \
+    case MM:                                                                     \
+        if (atten)                                                               \
+            launch_march_tw<MM, true>(tw, t, st, sp, step, air, I, P);           \
+        else                                                                     \
+            launch_march_tw<MM, false>(tw, t, st, sp, step, air, I, P);          \
         break;
         DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
 #undef DRR_CASE
@@ -131,11 +176,7 @@ static void batch_free(drr_batch *B) {
     if (B->cap) (void)hipStreamDestroy(B->cap);
     for (hipEvent_t e : {B->ev0, B->ev1, B->ev2})
         if (e) (void)hipEventDestroy(e);
-    (void)hipFree(B->d_jobs);
-    (void)hipFree(B->d_vols);
-    (void)hipFree(B->d_items);
-    (void)hipFree(B->d_seg);
-    (void)hipFree(B->d_counters);
+    drr_jobs_free(B->t);
     (void)hipFree(B->d_imgs);
     (void)hipFree(B->d_keys);
     delete B;
@@ -148,20 +189,33 @@ static hipError_t capture(drr_batch *B, SpectrumDev sp, float step, int air, flo
                           float *images, int neglog) {
     hipError_t e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    k_batch_reset<<<1, 64, 0, B->cap>>>(B->d_counters);
-    launch_march(B, B->cap, sp, step, air, I, P);
+    k_jobs_reset<<<1, 64, 0, B->cap>>>(B->t.d_counters);
+    launch_march(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P);
     e = hipStreamEndCapture(B->cap, &B->graph_march);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&B->exec_march, B->graph_march, nullptr, nullptr, 0);
     if (e != hipSuccess || !neglog) return e;
     e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    int rc = drr_enqueue_neglog(I, images, B->d_imgs, B->n_jobs, B->max_npix, B->d_keys, 0.01f, B->cap);
+    int rc = drr_enqueue_neglog(I, images, B->d_imgs, B->t.n_jobs, B->t.max_npix, B->d_keys, 0.01f, B->cap);
     hipError_t e2 = hipStreamEndCapture(B->cap, &B->graph_epi);
     if (rc != DRR_OK) return hipErrorLaunchFailure;
     if (e2 != hipSuccess) return e2;
     return hipGraphInstantiate(&B->exec_epi, B->graph_epi, nullptr, nullptr, 0);
 }
+
+namespace {
+struct DevSwitch {
+    int prev = -1, dev;
+    explicit DevSwitch(int d) : dev(d) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DevSwitch() {
+        if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
 
 extern "C" int drr_batch_create(drr_ctx *ctx, const drr_volume *const *vols, int n_vols,
                                 const drr_job_t *jobs, int n_jobs, const drr_batch_tuning_t *tuning,
@@ -171,116 +225,45 @@ extern "C" int drr_batch_create(drr_ctx *ctx, const drr_volume *const *vols, int
         return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: NULL argument");
     *out = nullptr;
     if (neglog && !images) return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: neglog needs images");
-    if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
-        return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: need >= 1 volume and 1..65535 jobs");
     if (!ctx->have_scene || ctx->n_bins < 1)
         return drr_set_error(DRR_ERR_STATE, "drr_batch_create: scene/spectrum not set");
-    drr_batch_tuning_t tune{8, 8, 4, 8};
-    if (tuning) tune = *tuning;
-    if (!((tune.tile_w == 8 && tune.tile_h == 8) || (tune.tile_w == 16 && tune.tile_h == 4)))
-        return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: tile must be 8x8 or 16x4");
-    if (tune.waves_per_block < 1 || tune.waves_per_block > 4 || tune.blocks_per_cu < 1 || tune.blocks_per_cu > 32)
-        return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: waves_per_block in [1,4], blocks_per_cu in [1,32]");
-    for (int v = 0; v < n_vols; v++) {
-        if (!vols[v] || vols[v]->ctx != ctx)
-            return drr_set_error(DRR_ERR_INVALID, "drr_batch_create: NULL or foreign volume");
-    }
-    int prev = -1;
-    (void)hipGetDevice(&prev);
-    if (prev != ctx->device) DRR_HIP_CHECK(hipSetDevice(ctx->device));
-
+    DevSwitch ds(ctx->device);
     drr_batch *B = new drr_batch();
     B->ctx = ctx;
-    B->n_jobs = n_jobs;
-    B->n_vols = n_vols;
     B->M = ctx->scene.n_materials;
     B->atten = ctx->scene.attenuate_outside != 0;
-    B->tune = tune;
-    int rc = DRR_OK;
-    auto fail = [&](int code) {
+    B->tune = tuning ? *tuning : drr_batch_tuning_t{8, 8, 4, 8};
+    int rc = drr_jobs_build(ctx, vols, n_vols, jobs, n_jobs, B->tune, B->t);
+    if (rc != DRR_OK) {
         batch_free(B);
-        if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
-        return code;
-    };
-
-    // Jobs and tiles.
-    std::vector<JobDev> hj(n_jobs);
-    std::vector<ImgSeg> himg(n_jobs);
-    std::vector<int> order(n_jobs);
-    for (int j = 0; j < n_jobs; j++) {
-        const drr_job_t &J = jobs[j];
-        if (J.view.n_volumes != 1)
-            return fail(drr_set_error(DRR_ERR_UNSUPPORTED, "drr_batch_create: each job must view exactly one volume"));
-        if (J.volume_index < 0 || J.volume_index >= n_vols)
-            return fail(drr_set_error(DRR_ERR_INVALID, "drr_batch_create: job volume_index out of range"));
-        if (J.view.width < 1 || J.view.height < 1 || J.out_offset < 0)
-            return fail(drr_set_error(DRR_ERR_INVALID, "drr_batch_create: bad job size/offset"));
-        view_to_geo1(J.view, hj[j].g);
-        hj[j].vol = J.volume_index;
-        hj[j].tiles_x = (J.view.width + tune.tile_w - 1) / tune.tile_w;
-        hj[j].out_offset = J.out_offset;
-        himg[j] = ImgSeg{J.out_offset, (int64_t)J.view.width * J.view.height};
-        B->max_npix = std::max<int64_t>(B->max_npix, himg[j].npix);
-        order[j] = j;
+        return rc;
     }
-    // Group tiles by volume (L2/MALL reuse across poses), then by job.
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hj[a].vol < hj[b].vol; });
-    std::vector<int2> items;
-    for (int j : order) {
-        const int ty = (jobs[j].view.height + tune.tile_h - 1) / tune.tile_h;
-        const int n = hj[j].tiles_x * ty;
-        for (int t = 0; t < n; t++) items.push_back(make_int2(j, t));
-    }
-    B->n_items = (int64_t)items.size();
-    if (B->n_items >= (int64_t)1 << 31)
-        return fail(drr_set_error(DRR_ERR_INVALID, "drr_batch_create: too many ray tiles"));
-    int seg[9];
-    for (int q = 0; q <= 8; q++) seg[q] = (int)(B->n_items * q / 8);
-    std::vector<VolDev> hv(n_vols);
-    for (int v = 0; v < n_vols; v++) hv[v] = vol_to_dev(vols[v]);
-
-    hipError_t e = hipSuccess;
+    hipError_t e;
 #define BCHK(x)                                                        \
     do {                                                               \
         e = (x);                                                       \
-        if (e != hipSuccess) return fail(drr_hip_error(e, #x));        \
+        if (e != hipSuccess) {                                         \
+            batch_free(B);                                             \
+            return drr_hip_error(e, #x);                               \
+        }                                                              \
     } while (0)
-    BCHK(hipMalloc(&B->d_jobs, sizeof(JobDev) * n_jobs));
-    BCHK(hipMalloc(&B->d_vols, sizeof(VolDev) * n_vols));
-    BCHK(hipMalloc(&B->d_items, sizeof(int2) * items.size()));
-    BCHK(hipMalloc(&B->d_seg, sizeof(int) * 9));
-    BCHK(hipMalloc(&B->d_counters, sizeof(int) * 8));
     BCHK(hipMalloc(&B->d_imgs, sizeof(ImgSeg) * n_jobs));
     BCHK(hipMalloc(&B->d_keys, sizeof(unsigned int) * 4 * n_jobs));
-    BCHK(hipMemcpy(B->d_jobs, hj.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
-    BCHK(hipMemcpy(B->d_vols, hv.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice));
-    BCHK(hipMemcpy(B->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
-    BCHK(hipMemcpy(B->d_seg, seg, sizeof(int) * 9, hipMemcpyHostToDevice));
-    BCHK(hipMemcpy(B->d_imgs, himg.data(), sizeof(ImgSeg) * n_jobs, hipMemcpyHostToDevice));
-    int cus = 0;
-    BCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    B->block = 64 * tune.waves_per_block;
-    B->grid = cus * tune.blocks_per_cu;
+    BCHK(hipMemcpy(B->d_imgs, B->t.imgs.data(), sizeof(ImgSeg) * n_jobs, hipMemcpyHostToDevice));
     BCHK(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking));
     BCHK(hipEventCreate(&B->ev0));
     BCHK(hipEventCreate(&B->ev1));
     BCHK(hipEventCreate(&B->ev2));
-#undef BCHK
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
-    const float step = ctx->scene.step;
-    const int air = ctx->scene.air_index;
-    e = capture(B, sp, step, air, intensity, photon_prob, images, neglog);
-    if (e != hipSuccess) return fail(drr_hip_error(e, "drr_batch_create: graph capture"));
-    if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
+    BCHK(capture(B, sp, ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, images, neglog));
+#undef BCHK
     *out = B;
-    return rc;
+    return DRR_OK;
 }
 
 extern "C" int drr_batch_launch(drr_batch *B, void *stream) {
     if (!B) return drr_set_error(DRR_ERR_INVALID, "drr_batch_launch: NULL batch");
-    int prev = -1;
-    (void)hipGetDevice(&prev);
-    if (prev != B->ctx->device) DRR_HIP_CHECK(hipSetDevice(B->ctx->device));
+    DevSwitch ds(B->ctx->device);
     hipStream_t st = (hipStream_t)stream;
     DRR_HIP_CHECK(hipEventRecord(B->ev0, st));
     DRR_HIP_CHECK(hipGraphLaunch(B->exec_march, st));
@@ -288,7 +271,6 @@ extern "C" int drr_batch_launch(drr_batch *B, void *stream) {
     if (B->exec_epi) DRR_HIP_CHECK(hipGraphLaunch(B->exec_epi, st));
     DRR_HIP_CHECK(hipEventRecord(B->ev2, st));
     B->launched = true;
-    if (prev >= 0 && prev != B->ctx->device) (void)hipSetDevice(prev);
     return DRR_OK;
 }
 
@@ -302,12 +284,35 @@ extern "C" int drr_batch_last_march_ms(drr_batch *B, float *ms) {
 
 extern "C" int drr_batch_destroy(drr_batch *B) {
     if (!B) return DRR_OK;
-    int prev = -1;
-    (void)hipGetDevice(&prev);
-    if (prev != B->ctx->device) (void)hipSetDevice(B->ctx->device);
+    DevSwitch ds(B->ctx->device);
     if (B->launched) (void)hipEventSynchronize(B->ev2);
-    const int dev = B->ctx->device;
     batch_free(B);
-    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    return DRR_OK;
+}
+
+extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                                int n_jobs, float *intensity, float *photon_prob, void *stream) {
+    if (!ctx || !vols || !jobs || !intensity)
+        return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs: NULL argument");
+    if (!ctx->have_scene || ctx->n_bins < 1)
+        return drr_set_error(DRR_ERR_STATE, "drr_project_jobs: scene/spectrum not set");
+    DevSwitch ds(ctx->device);
+    JobTable t;
+    const drr_batch_tuning_t tune{8, 8, 4, 8};
+    int rc = drr_jobs_build(ctx, vols, n_vols, jobs, n_jobs, tune, t);
+    if (rc != DRR_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
+    (void)hipEventRecord(ctx->ev0, st);
+    k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);
+    launch_march(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
+                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob);
+    hipError_t e = hipGetLastError();
+    (void)hipEventRecord(ctx->ev1, st);
+    ctx->timed = true;
+    hipError_t e2 = hipStreamSynchronize(st);
+    drr_jobs_free(t);
+    if (e != hipSuccess) return drr_hip_error(e, "drr_project_jobs launch");
+    if (e2 != hipSuccess) return drr_hip_error(e2, "drr_project_jobs");
     return DRR_OK;
 }

@@ -143,6 +143,8 @@ inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, View
 
 // Validation shared by the entry points.
 int drr_check_view(const drr_ctx *ctx, const drr_volume *const *vols, const drr_view_t *view);
+// Every float of the view's geometry is finite (else DRR_ERR_INVALID).
+int drr_check_view_finite(const drr_view_t *view);
 
 // Make vol->lin_dens / lin_lab / max_density if absent (runtime.hip).
 int drr_volume_ensure_linear(drr_volume *vol);

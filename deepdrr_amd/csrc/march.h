@@ -71,8 +71,9 @@ __device__ __forceinline__ bool slab(const float *A, const float *s, const float
 }
 
 __device__ __forceinline__ int clamp_cell(float b, int n) {
-    // cell index in [-1, n-1]; clamp in float so huge/NaN values stay defined
-    const float c = fminf(fmaxf(b, -1.0f), (float)(n - 1));
+    // cell index in [-1, n-1]; clamp in float (one v_med3_f32) so that huge
+    // values stay defined before the conversion
+    const float c = __builtin_amdgcn_fmed3f(b, -1.0f, (float)(n - 1));
     return (int)c;
 }
 
@@ -318,10 +319,15 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
             const float pz = sz + alpha * riz - 1.0f;
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
             if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
-                if (cm >= 0) set_m<M>(ad, cm, cur);
                 c.load(vol, px, py, pz, bx, by, bz);
-                cm = c.uni ? (int)c.m0 : -1;
-                cur = get_m<M>(ad, cm);
+                // ad[cm] lives in `cur` across consecutive uniform cells of one
+                // material; it is written back only when the material changes.
+                const int ncm = c.uni ? (int)c.m0 : -1;
+                if (ncm != cm) {
+                    if (cm >= 0) set_m<M>(ad, cm, cur);
+                    cm = ncm;
+                    cur = get_m<M>(ad, cm);
+                }
             }
             float w[8];
             weights8(px - bx, py - by, pz - bz, w);

@@ -398,5 +398,19 @@ int drr_check_view(const drr_ctx *ctx, const drr_volume *const *vols, const drr_
         if (!vols || !vols[i]) return drr_set_error(DRR_ERR_INVALID, "NULL volume in view");
         if (vols[i]->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "volume belongs to another context");
     }
-    return DRR_OK;
+    return drr_check_view_finite(view);
+}
+
+int drr_check_view_finite(const drr_view_t *view) {
+    // the march's cell indices are clamped from float sample positions, which
+    // stay finite (and in range after the clamp) only for finite geometry
+    bool ok = std::isfinite(view->max_ray_length);
+    for (int i = 0; i < 9; i++) ok = ok && std::isfinite(view->world_from_index[i]);
+    for (int v = 0; v < view->n_volumes && v < DRR_MAX_VOLUMES; v++) {
+        const drr_view_volume_t &q = view->vol[v];
+        for (int i = 0; i < 12; i++) ok = ok && std::isfinite(q.ijk_from_world[i]);
+        for (int i = 0; i < 3; i++)
+            ok = ok && std::isfinite(q.source_ijk[i]) && std::isfinite(q.bounds_lo[i]) && std::isfinite(q.bounds_hi[i]);
+    }
+    return ok ? DRR_OK : drr_set_error(DRR_ERR_INVALID, "view geometry must be finite");
 }

@@ -144,8 +144,8 @@ typedef struct {
     int32_t blocks_per_cu;     /* persistent blocks per CU */
 } drr_batch_tuning_t;
 
-/* Build a batch plan: uploads the job table, sorts ray tiles longest-first,
- * and captures [queue reset, persistent march, optional neglog epilogue]
+/* Build a batch plan: uploads the job table, groups ray tiles by volume into
+ * 8 per-XCD work queues (with stealing), and captures [queue reset, persistent march, optional neglog epilogue]
  * into a hipGraph.  `neglog` = 1 appends the device neglog of each image
  * (image_utils.py:18-59) writing `images` (float32, same offsets). */
 int drr_batch_create(drr_ctx *ctx, const drr_volume *const *vols, int n_vols,
@@ -157,6 +157,13 @@ int drr_batch_launch(drr_batch *batch, void *stream);
 /* Kernel-only time (ms) of the last launch's persistent march, from HIP events. */
 int drr_batch_last_march_ms(drr_batch *batch, float *ms);
 int drr_batch_destroy(drr_batch *batch);
+
+/* One-shot forms of the batch (no graph, job tables built and freed per call;
+ * the call returns after the launch completes).  drr_project_jobs writes each
+ * job's intensity / photon_prob image at its out_offset (photon_prob may be
+ * NULL). */
+int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                     int n_jobs, float *intensity, float *photon_prob, void *stream);
 
 /* ---- device epilogue --------------------------------------------------------- */
 
@@ -192,6 +199,14 @@ int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t *view,
                 const float *grad_intensity, const float *grad_photon_prob,
                 const float *source_world, const float *pose_center_world,
                 float *grad_density, float *grad_pose, void *stream);
+
+/* drr_adjoint for many views of one volume in one persistent launch (jobs as
+ * in drr_batch_create, all with volume_index 0; the gradient images are read
+ * at each job's out_offset).  source_world: 3 floats per job (host). */
+int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_job_t *jobs, int n_jobs,
+                     const float *grad_intensity, const float *grad_photon_prob,
+                     const float *source_world, const float *pose_center_world,
+                     float *grad_density, float *grad_pose, void *stream);
 
 /* ---- Monte-Carlo scatter ----------------------------------------------------- */
 

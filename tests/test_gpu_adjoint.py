@@ -200,3 +200,59 @@ def test_adjoint_fd_single_material_smooth(gpu):
         assert errs[1] < 1e-3 and errs[1] < errs[0]
     finally:
         dp.close()
+
+
+def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
+    """drr_adjoint_jobs (one persistent launch) == sum over views of drr_adjoint,
+    up to float atomic ordering; drr_project_jobs == drr_project bit for bit."""
+    import ctypes as C
+
+    from deepdrr_amd import _abi
+    from deepdrr_amd._abi_types import Job
+
+    dens, labels, W, projs, step = _scene(n=18, n_views=4, det=14, seed=5)
+    dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
+    try:
+        views = dp.views(projs, np.zeros(6))
+        H, Wd = views[0][0].height, views[0][0].width
+        d_kji = torch.tensor(np.ascontiguousarray(np.moveaxis(dens, [0, 1, 2], [2, 1, 0])), device=gpu)
+        h = dp.upload(d_kji)
+        varr = (C.c_void_p * 1)(h.value)
+        jobs = (Job * len(views))()
+        for i, (v, _) in enumerate(views):
+            jobs[i].view = v
+            jobs[i].volume_index = 0
+            jobs[i].out_offset = i * H * Wd
+        I_j = torch.empty((len(views), H, Wd), device=gpu)
+        P_j = torch.empty_like(I_j)
+        _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I_j), _abi.ptr(P_j),
+                  _abi.stream_ptr())
+        rng = np.random.default_rng(2)
+        gI = torch.tensor(rng.standard_normal((len(views), H, Wd)).astype(np.float32), device=gpu)
+        gPp = torch.tensor(rng.standard_normal((len(views), H, Wd)).astype(np.float32), device=gpu)
+        c = np.asarray(dp.center, np.float32)
+        src = np.ascontiguousarray(np.stack([s for _, s in views]), np.float32)
+        gD_j = torch.zeros_like(d_kji)
+        gT_j = torch.zeros(6, device=gpu)
+        _abi.call("drr_adjoint_jobs", dp.ctx, h, jobs, len(views), _abi.ptr(gI), _abi.ptr(gPp),
+                  src.ctypes.data, c.ctypes.data, _abi.ptr(gD_j), _abi.ptr(gT_j), _abi.stream_ptr())
+        gD_s = torch.zeros_like(d_kji)
+        gT_s = torch.zeros(6, device=gpu)
+        for i, (v, s) in enumerate(views):
+            I = torch.empty((H, Wd), device=gpu)
+            Pp = torch.empty_like(I)
+            _abi.call("drr_project", dp.ctx, varr, C.byref(v), _abi.ptr(I), _abi.ptr(Pp), None, None,
+                      _abi.stream_ptr())
+            torch.cuda.synchronize()
+            assert torch.equal(I, I_j[i]) and torch.equal(Pp, P_j[i]), f"view {i}"
+            _abi.call("drr_adjoint", dp.ctx, h, C.byref(v), _abi.ptr(gI[i]), _abi.ptr(gPp[i]),
+                      s.ctypes.data, c.ctypes.data, _abi.ptr(gD_s), _abi.ptr(gT_s), _abi.stream_ptr())
+        torch.cuda.synchronize()
+        _abi.lib().drr_volume_destroy(h)
+        a, b = gD_j.cpu().numpy(), gD_s.cpu().numpy()
+        assert np.abs(a).max() > 0
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+        np.testing.assert_allclose(gT_j.cpu().numpy(), gT_s.cpu().numpy(), rtol=1e-4,
+                                   atol=1e-5 * float(np.abs(gT_s.cpu().numpy()).max()))
+    finally:
+        dp.close()
