@@ -219,8 +219,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--tuning", type=int, nargs=4, default=[8, 8, 4, 8],
-                    help="tile_w tile_h waves_per_block blocks_per_cu")
+    ap.add_argument("--tuning", type=int, nargs=4, default=None,
+                    help="tile_w tile_h waves_per_block blocks_per_cu (default: autotuned)")
     ap.add_argument("--step-mm", type=float, default=0.1)
     ap.add_argument("--plain-views", type=int, default=3)
     ap.add_argument("--mc-histories", type=int, default=100_000_000)
@@ -287,7 +287,14 @@ def main():
     views = [P.build_view([geom], [0], phantoms.pose_projection(1536, 1536, 0.194, p, d, f), 4 * 1020.0)
              for p, d, f in phantoms.batch_poses(B, seed=0)]
     samples_per_launch = sum(workload.count_samples(v, args.step_mm, stride=2) for v in views)
-    plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True, tuning=tuple(args.tuning))
+    from deepdrr_amd import autotune
+
+    tune_key = autotune.workload_key(views, [0] * B, [(512, 512, 400)], len(materials),
+                                     torch.cuda.get_device_name(0))
+    t = time.perf_counter()
+    plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True,
+                     tuning=tuple(args.tuning) if args.tuning else "auto", tune_key=tune_key)
+    log(f"schedule {plan.tuning} ({'given' if args.tuning else 'autotuned'}, {time.perf_counter() - t:.1f} s)")
     for _ in range(args.warmup):
         plan.launch()
         plan.fetch("images")
@@ -315,11 +322,13 @@ def main():
                             "U(0.5,0.9)) of a 512x512x400 int16 CT (0.8x0.8x1 mm, 5 materials) -> "
                             "1536x1536 detector at 0.194 mm, SDD 1020 mm, step 0.1 mm, 20-bin 120 kV, "
                             "device neglog, one persistent launch per step replayed from a hipGraph",
-                "views_per_step": B, "tuning": list(args.tuning)},
+                "views_per_step": B, "tuning": list(plan.tuning),
+                "tuning_source": "given" if args.tuning else "autotuned"},
         plain_views_per_s=plain_vps, plain_march_ms_per_view=plain_kernel_ms,
         batched_march_ms_per_view=kernel_ms / B, batched_over_plain=value / plain_vps,
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_march<5,false,8>",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "kernel": f"k_batch_march<{len(materials)},false,{plan.tuning[0]}>",
                   "kernel_ms": kernel_ms, "samples_per_launch": samples_per_launch,
                   "bytes_per_sample": workload.BYTES_PER_SAMPLE,
                   "samples_per_s": samples_per_launch / (kernel_ms * 1e-3)},

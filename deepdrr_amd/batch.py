@@ -27,10 +27,15 @@ class BatchPlan:
     ``vol_handles``.  Outputs live in device tensors ``intensity``,
     ``photon_prob`` and (with ``neglog``) ``images``, each image at
     ``offsets[i]`` with shape ``shapes[i]``.
+
+    ``tuning``: a ``(tile_w, tile_h, waves_per_block, blocks_per_cu)`` tuple,
+    None for the default, or ``"auto"`` to time the candidates of
+    ``deepdrr_amd.autotune`` on a sample of these views first (cached under
+    ``tune_key``, see ``autotune.workload_key``).
     """
 
     def __init__(self, ctx, vol_handles: Sequence, views: Sequence, volume_indices: Sequence[int],
-                 neglog: bool = True, tuning=None, device=None):
+                 neglog: bool = True, tuning=None, device=None, tune_key: Optional[str] = None):
         import torch
 
         self.torch = torch
@@ -40,6 +45,16 @@ class BatchPlan:
             raise ValueError("empty batch")
         if len(volume_indices) != n:
             raise ValueError("one volume index per view is needed")
+        if isinstance(tuning, str):
+            if tuning != "auto":
+                raise ValueError("tuning must be a 4-tuple, None or 'auto'")
+            from . import autotune
+
+            tuning = autotune.tune(ctx, vol_handles, views, volume_indices,
+                                   tune_key or autotune.workload_key(views, volume_indices,
+                                                                     [(0, 0, 0)] * len(vol_handles), 0),
+                                   device=device)
+        self.tuning = tuple(tuning or DEFAULT_TUNING)
         self.shapes = [(v.height, v.width) for v in views]
         sizes = [h * w for h, w in self.shapes]
         self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
@@ -55,7 +70,7 @@ class BatchPlan:
         self.photon_prob = torch.empty(total, dtype=torch.float32, device=self.device)
         self.neglog = bool(neglog)
         self.images = torch.empty(total, dtype=torch.float32, device=self.device) if neglog else None
-        t = BatchTuning(*(tuning or DEFAULT_TUNING))
+        t = BatchTuning(*self.tuning)
         self._varr = (C.c_void_p * len(vol_handles))(*[h.value if hasattr(h, "value") else h
                                                          for h in vol_handles])
         self._batch = C.c_void_p()
@@ -119,6 +134,12 @@ def plan_for_projector(projector, projections: Sequence[geo.CameraProjection],
     for proj, vi in zip(projections, volume_indices):
         views.append(build_view([projector.volumes[vi]], [projector.priorities[vi]], proj,
                                 projector.max_ray_length, sdd if sdd > 0 else 1.0))
+    key = None
+    if tuning == "auto":
+        from . import autotune
+
+        key = autotune.workload_key(views, volume_indices, [v.shape for v in projector.volumes],
+                                    len(projector.all_materials))
     return BatchPlan(projector._ctx, projector._vols, views, volume_indices,
                      neglog=projector.neglog if neglog is None else neglog, tuning=tuning,
-                     device=projector._torch_device)
+                     device=projector._torch_device, tune_key=key)

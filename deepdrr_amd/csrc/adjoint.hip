@@ -66,10 +66,7 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
     const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
     const float rdom = fmaxf(fabsf(rix), fmaxf(fabsf(riy), fabsf(riz)));
     const float lim = r.mnv - (1.0f / rdom + 2.0f * step);
-    while (t < N && alpha < lim) {
-        alpha += step;
-        ++t;
-    }
+    skip_steps(N, lim, step, alpha, t);
     Cell c;
     while (t < N && alpha < r.mnv) {
         const float px = sx + alpha * rix - 1.0f;

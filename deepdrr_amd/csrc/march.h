@@ -239,6 +239,24 @@ struct RayV1 {
     bool trace;
 };
 
+// `while (t < N && alpha < lim) { alpha += step; ++t; }` with the same FP32
+// additions, tested once per 8 steps: alpha only grows, so the 8 tests of a
+// block pass exactly when the one before its last addition does.
+__device__ __forceinline__ void skip_steps(int N, float lim, float step, float &alpha, int &t) {
+    while (t + 8 <= N) {
+        float a = alpha;
+#pragma unroll
+        for (int k = 0; k < 7; k++) a += step;
+        if (!(a < lim)) break;
+        alpha = a + step;
+        t += 8;
+    }
+    while (t < N && alpha < lim) {
+        alpha += step;
+        ++t;
+    }
+}
+
 __device__ __forceinline__ void march_setup_v1(const Geo1 &g, int udx, int vdx, float step, RayV1 &r) {
     ray_dir(g.W, udx, vdx, r.rx, r.ry, r.rz, r.L);
     r.minA = r.L;
@@ -285,10 +303,7 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
                 ++t;
             }
         } else {
-            while (t < N && alpha < lim) {
-                alpha += step;
-                ++t;
-            }
+            skip_steps(N, lim, step, alpha, t);
         }
         // Phase 2: approach window, label cache tracked as the reference does.
         Cell c;

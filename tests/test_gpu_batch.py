@@ -145,3 +145,28 @@ def test_projector_batch_dropin(gpu):
         ref = proj(*projs)
         got = proj.project_batch(*projs)
     np.testing.assert_allclose(got, ref, atol=1e-5)
+
+
+def test_autotuned_plan_gives_identical_images(gpu):
+    """tuning='auto' picks one of the candidates and only changes the schedule."""
+    import torch
+
+    from deepdrr_amd import autotune
+
+    ctx, handles, arrays, views, vidx, sc, E, pdf, mu = _setup(n_views=9, seed=2)
+    try:
+        key = autotune.workload_key(views, vidx, [a[0].shape[::-1] for a in arrays], sc.n_materials, "test")
+        auto = BatchPlan(ctx, handles, views, vidx, neglog=False, tuning="auto", tune_key=key)
+        print(f"autotuned schedule {auto.tuning}", flush=True)
+        assert auto.tuning in autotune.CANDIDATES
+        assert autotune.cached(key) == auto.tuning
+        ref = BatchPlan(ctx, handles, views, vidx, neglog=False, tuning=(8, 8, 4, 8))
+        auto.launch()
+        ref.launch()
+        torch.cuda.synchronize()
+        for a, b in zip(auto.fetch("intensity"), ref.fetch("intensity")):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        auto.close()
+        ref.close()
+    finally:
+        _teardown(ctx, handles)
