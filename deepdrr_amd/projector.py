@@ -145,6 +145,9 @@ class Projector:
 
         self.add_noise = add_noise
         self.photon_count = photon_count
+        # seed of the device noise of project_batch (None: a fresh seed per call,
+        # as the reference draws from an unseeded generator)
+        self.noise_seed: Optional[int] = None
         self.threads = threads
         self.max_block_index = max_block_index
         self.collected_energy = collected_energy
@@ -365,12 +368,20 @@ class Projector:
         from .batch import plan_for_projector
 
         camera_projections = self._prepare_project(camera_projections)
-        host_epilogue = self.add_noise or self.intensity_upper_bound is not None or self.scatter_num > 0
+        sizes = {tuple(p.intrinsic.sensor_size) for p in camera_projections}
+        # noise and neglog run on the device (one D2H copy) when every image has
+        # one size and nothing needs the host (scatter, intensity clip)
+        device_noise = (self.add_noise and len(sizes) == 1 and self.scatter_num == 0
+                        and self.intensity_upper_bound is None)
+        host_epilogue = (self.add_noise and not device_noise) or self.intensity_upper_bound is not None \
+            or self.scatter_num > 0
         plan = plan_for_projector(self, camera_projections, volume_indices,
-                                  neglog=self.neglog and not host_epilogue, tuning=tuning)
+                                  neglog=self.neglog and not host_epilogue and not device_noise, tuning=tuning)
         try:
             plan.launch()
-            if host_epilogue:
+            if device_noise:
+                out = self._device_noise_epilogue(plan)
+            elif host_epilogue:
                 images = plan.fetch("intensity")
                 photon_prob = plan.fetch("photon_prob")
                 out = []
@@ -398,6 +409,24 @@ class Projector:
         if len({o.shape for o in out}) == 1:
             return np.stack(out)
         return out
+
+    def _device_noise_epilogue(self, plan) -> List[np.ndarray]:
+        """drr_add_noise then drr_neglog over the batch's equal-size images."""
+        import torch
+
+        h, w = plan.shapes[0]
+        n = plan.n_views
+        noisy = torch.empty_like(plan.intensity)
+        stream = _abi.stream_ptr(torch.cuda.current_stream(plan.device))
+        seed = self.noise_seed if getattr(self, "noise_seed", None) is not None else \
+            int(np.random.default_rng().integers(0, 2 ** 63))
+        _abi.call("drr_add_noise", self._ctx, _abi.ptr(plan.intensity), _abi.ptr(plan.photon_prob),
+                  _abi.ptr(noisy), n, h, w, C.c_float(self.photon_count), C.c_uint64(seed), stream)
+        if self.neglog:
+            _abi.call("drr_neglog", self._ctx, _abi.ptr(noisy), _abi.ptr(noisy), n, h * w,
+                      C.c_float(utils.NEGLOG_EPSILON), stream)
+        host = noisy.cpu().numpy()
+        return [host[i * h * w:(i + 1) * h * w].reshape(h, w) for i in range(n)]
 
     def project_area_density(self, proj: geo.CameraProjection):
         """Per-material area densities [M, H, W] (g/cm^2), intensity and photon_prob of one view."""

@@ -20,7 +20,10 @@ KERNEL_ELECTRONIC_NOISE = np.array([[0.0, 0.0, 0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0
                                     [0.0, 0.0, 0.0, 0.0, 0.0]])
 
 
-def neglog(image: np.ndarray, epsilon: float = 0.01) -> np.ndarray:
+NEGLOG_EPSILON = 0.01  # image_utils.py:18 default
+
+
+def neglog(image: np.ndarray, epsilon: float = NEGLOG_EPSILON) -> np.ndarray:
     """x += min(x) + eps; y = -log(x); min-max normalise per image (0 if constant)."""
     image = np.array(image)
     shape = image.shape

@@ -170,3 +170,30 @@ def test_autotuned_plan_gives_identical_images(gpu):
         ref.close()
     finally:
         _teardown(ctx, handles)
+
+
+def test_projector_batch_device_noise(gpu):
+    """With add_noise, project_batch runs noise and neglog on the device: seeded
+    calls repeat, the images stay in [0, 1] and differ from the noise-free ones,
+    and the noisy intensity keeps the clean mean (zero-mean shot noise)."""
+    from deepdrr_amd import Projector
+
+    vol = phantoms.small_ct(24)
+    dev = phantoms.ap_device(48, 40, 1.0)
+    projs = [phantoms.pose_projection(48, 40, 1.0, p, d, f)
+             for p, d, f in phantoms.batch_poses(4, seed=3, radius=3.0)]
+    with Projector(vol, device=dev, spectrum=synthetic_120kv(), add_noise=True, photon_count=2000) as proj:
+        proj.noise_seed = 11
+        a = proj.project_batch(*projs)
+        b = proj.project_batch(*projs)
+        proj.neglog = False
+        raw = proj.project_batch(*projs)
+        proj.add_noise = False
+        clean_raw = proj.project_batch(*projs)
+        proj.neglog = True
+        clean = proj.project_batch(*projs)
+    assert a.shape == (4, 40, 48) and np.array_equal(a, b)
+    assert np.all(a >= 0) and np.all(a <= 1) and not np.array_equal(a, clean)
+    inner = (slice(None), slice(2, -2), slice(2, -2))
+    rel = abs(raw[inner].mean() - clean_raw[inner].mean()) / clean_raw[inner].mean()
+    assert rel < 0.01, rel
