@@ -57,6 +57,25 @@ def make_ctx(_abi, materials, step):
     return ctx, (E, pdf, mu, sc)
 
 
+def profiled_traffic(views_per_launch):
+    """HBM bytes per batched-march launch measured by rocprofv3 on this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy-kernel calibration), read
+    from the committed counter summary; None when it is absent or was taken
+    at another batch size.  Returns (bytes, schedule the counters were taken with)."""
+    path = os.path.join(ROOT, "profiles", "k_batch_march_clinical_1536_pmc.md")
+    if not os.path.exists(path):
+        return None, None
+    vals = {}
+    for ln in open(path):
+        ln = ln.strip()
+        for key in ("traffic_bytes_per_launch", "views_per_launch", "tuning"):
+            if ln.startswith(key + ":"):
+                vals[key] = ln.split(":", 1)[1].strip()
+    if "traffic_bytes_per_launch" not in vals or int(vals.get("views_per_launch", -1)) != views_per_launch:
+        return None, None
+    return float(vals["traffic_bytes_per_launch"]), [int(x) for x in vals.get("tuning", "").split()]
+
+
 def plain_views(torch, _abi, ctx, varr, views, dev, utils):
     """The reference's per-view flow; returns images [N,H,W] and kernel ms per view."""
     imgs, kms = [], []
@@ -311,6 +330,7 @@ def main():
     value = B * args.steps / dt
     kernel_ms = float(np.mean(kms))
     achieved = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_tuning = profiled_traffic(B)
     log(f"batched: {value:.2f} views/s, march {kernel_ms / B:.2f} ms/view")
 
     result = dict(
@@ -327,7 +347,10 @@ def main():
         plain_views_per_s=plain_vps, plain_march_ms_per_view=plain_kernel_ms,
         batched_march_ms_per_view=kernel_ms / B, batched_over_plain=value / plain_vps,
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                  "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated, schedule "
+                                     f"{traffic_tuning} (profiles/k_batch_march_clinical_1536_pmc.md)"
+                                     if traffic is not None else None),
                   "kernel": f"k_batch_march<{len(materials)},false,{plan.tuning[0]}>",
                   "kernel_ms": kernel_ms, "samples_per_launch": samples_per_launch,
                   "bytes_per_sample": workload.BYTES_PER_SAMPLE,

@@ -156,17 +156,23 @@ def _make_function():
             h = dp.upload(dens_kji)
             varr = (C.c_void_p * 1)(h.value)
             I = torch.empty((len(views), H, W), dtype=torch.float32, device=dp.device)
+            need_grad = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
+            # the area densities of the forward march, reused by the backward pass
+            AD = (torch.empty((len(views), len(dp.materials), H, W), dtype=torch.float32, device=dp.device)
+                  if need_grad else None)
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
             try:
-                _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I), None, stream)
+                _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I), None,
+                          _abi.ptr(AD), stream)
             except Exception:
                 _abi.lib().drr_volume_destroy(h)
                 raise
+            ctx.AD = AD
             ctx.dp = dp
             ctx.jobs = jobs
             ctx.sources = np.ascontiguousarray(np.stack([src for _, src in views]), np.float32)
             ctx.vol = h
-            if not (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]):
+            if not need_grad:
                 _abi.lib().drr_volume_destroy(h)
                 ctx.vol = None
             ctx.shape = density.shape
@@ -184,10 +190,12 @@ def _make_function():
             c = np.asarray(dp.center, np.float32)
             try:
                 _abi.call("drr_adjoint_jobs", dp.ctx, ctx.vol, ctx.jobs, len(ctx.jobs), _abi.ptr(grad_out),
-                          None, ctx.sources.ctypes.data, c.ctypes.data, _abi.ptr(gD), _abi.ptr(gP), stream)
+                          None, ctx.sources.ctypes.data, c.ctypes.data, _abi.ptr(ctx.AD), _abi.ptr(gD),
+                          _abi.ptr(gP), stream)
             finally:
                 _abi.lib().drr_volume_destroy(ctx.vol)
                 ctx.vol = None
+                ctx.AD = None
             grad_theta = gP if ctx.need_pose else None
             return gD.permute(2, 1, 0), grad_theta, None, None
 

@@ -35,75 +35,167 @@ __device__ __forceinline__ void grad_trilinear(const float v[8], float fx, float
              (v[7] - v[3]) * (fx * fy);
 }
 
-__device__ __forceinline__ void flush_cell(const VolDev &vol, float *gD, float bx, float by, float bz,
-                                           float acc[8]) {
-    // cell (cx, cy, cz) in [-1, n-1]^3; its corners clamp to [0, n-1] (voxels < 2^31)
-    const int cx = clamp_cell(bx, vol.nx), cy = clamp_cell(by, vol.ny), cz = clamp_cell(bz, vol.nz);
-    const uint32_t x0 = max(cx, 0), x1 = min(cx + 1, vol.nx - 1);
-    const uint32_t y0 = max(cy, 0), y1 = min(cy + 1, vol.ny - 1);
-    const uint32_t z0 = max(cz, 0), z1 = min(cz + 1, vol.nz - 1);
-    const uint32_t nx = vol.nx, ny = vol.ny;
-    const uint32_t row[4] = {(z0 * ny + y0) * nx, (z0 * ny + y1) * nx, (z1 * ny + y0) * nx, (z1 * ny + y1) * nx};
-#pragma unroll
-    for (int d = 0; d < 8; d++) {
-        if (acc[d] != 0.0f) unsafeAtomicAdd(gD + (row[d >> 1] + ((d & 1) ? x1 : x0)), acc[d]);
-        acc[d] = 0.0f;
-    }
-}
+// ---- where the per-cell corner sums go ----------------------------------------
 
-// Pass 2 of one ray: scatter density gradients, accumulate pose gradients.
-template <int M, bool POSE>
-__device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, int udx, int vdx,
-                                              float step, const float (&graw)[M], const float sw_world[3],
-                                              const float center[3], float *gD, double (&gpose)[6]) {
-    RayV1 r;
-    march_setup_v1(g, udx, vdx, step, r);
-    if (!r.trace) return;
-    const int N = r.N;
-    float alpha = r.minA;
-    int t = 0;
-    const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
-    const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
-    const float rdom = fmaxf(fabsf(rix), fmaxf(fabsf(riy), fabsf(riz)));
-    const float lim = r.mnv - (1.0f / rdom + 2.0f * step);
-    skip_steps(N, lim, step, alpha, t);
-    Cell c;
-    while (t < N && alpha < r.mnv) {
-        const float px = sx + alpha * rix - 1.0f;
-        const float py = sy + alpha * riy - 1.0f;
-        const float pz = sz + alpha * riz - 1.0f;
-        const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-        if (!c.have || bx != c.bx || by != c.by || bz != c.bz) c.load(vol, px, py, pz, bx, by, bz);
-        alpha += step;
-        ++t;
+// Clamped corner voxels of cell (bx, by, bz) (cells in [-1, n-1]^3, voxels < 2^31).
+struct CellCorners {
+    uint32_t x0, x1, y0, y1, z0, z1;
+    __device__ __forceinline__ CellCorners(const VolDev &vol, float bx, float by, float bz) {
+        const int cx = clamp_cell(bx, vol.nx), cy = clamp_cell(by, vol.ny), cz = clamp_cell(bz, vol.nz);
+        x0 = max(cx, 0);
+        x1 = min(cx + 1, vol.nx - 1);
+        y0 = max(cy, 0);
+        y1 = min(cy + 1, vol.ny - 1);
+        z0 = max(cz, 0);
+        z1 = min(cz + 1, vol.nz - 1);
     }
+};
+
+// Straight to the gradient volume with float atomics.
+struct GlobalDep {
+    float *gD;
+    __device__ __forceinline__ void operator()(const VolDev &vol, float bx, float by, float bz, float (&acc)[8]) {
+        const CellCorners k(vol, bx, by, bz);
+        const uint32_t nx = vol.nx, ny = vol.ny;
+        const uint32_t row[4] = {(k.z0 * ny + k.y0) * nx, (k.z0 * ny + k.y1) * nx, (k.z1 * ny + k.y0) * nx,
+                                 (k.z1 * ny + k.y1) * nx};
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            if (acc[d] != 0.0f) unsafeAtomicAdd(gD + (row[d >> 1] + ((d & 1) ? k.x1 : k.x0)), acc[d]);
+            acc[d] = 0.0f;
+        }
+    }
+};
+
+// Into a workgroup's LDS box [o, o + L) (voxels, per axis); corners outside it
+// go to the gradient volume directly, so the box only changes the speed.
+struct BoxDep {
+    float *box;
+    int ox, oy, oz, Lx, Ly, Lz;
+    float *gD;
+    __device__ __forceinline__ void operator()(const VolDev &vol, float bx, float by, float bz, float (&acc)[8]) {
+        const CellCorners k(vol, bx, by, bz);
+        const int ix[2] = {(int)k.x0 - ox, (int)k.x1 - ox}, iy[2] = {(int)k.y0 - oy, (int)k.y1 - oy},
+                  iz[2] = {(int)k.z0 - oz, (int)k.z1 - oz};
+        // x0 <= x1 etc., so the cell is inside when both ends are
+        const bool inside = ((uint32_t)ix[0] < (uint32_t)Lx) & ((uint32_t)ix[1] < (uint32_t)Lx) &
+                            ((uint32_t)iy[0] < (uint32_t)Ly) & ((uint32_t)iy[1] < (uint32_t)Ly) &
+                            ((uint32_t)iz[0] < (uint32_t)Lz) & ((uint32_t)iz[1] < (uint32_t)Lz);
+        if (inside) {
+            const int row[4] = {(iz[0] * Ly + iy[0]) * Lx, (iz[0] * Ly + iy[1]) * Lx, (iz[1] * Ly + iy[0]) * Lx,
+                                (iz[1] * Ly + iy[1]) * Lx};
+#pragma unroll
+            for (int d = 0; d < 8; d++) atomicAdd(box + (row[d >> 1] + ix[d & 1]), acc[d]);
+        } else {
+            const uint32_t nx = vol.nx, ny = vol.ny;
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                const int a = d & 1, b = (d >> 1) & 1, c = d >> 2;
+                if (acc[d] == 0.0f) continue;
+                const bool in = ((uint32_t)ix[a] < (uint32_t)Lx) & ((uint32_t)iy[b] < (uint32_t)Ly) &
+                                ((uint32_t)iz[c] < (uint32_t)Lz);
+                if (in) {
+                    atomicAdd(box + ((iz[c] * Ly + iy[b]) * Lx + ix[a]), acc[d]);
+                } else {
+                    const uint32_t x = a ? k.x1 : k.x0, y = b ? k.y1 : k.y0, z = c ? k.z1 : k.z0;
+                    unsafeAtomicAdd(gD + ((z * ny + y) * nx + x), acc[d]);
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 8; d++) acc[d] = 0.0f;
+    }
+};
+
+// Which cells a call of AdjRay::run may enter: all, or those up to the end of
+// the current slab (along the slab axis, in march order).
+struct AllCells {
+    __device__ __forceinline__ bool operator()(int) const { return true; }
+};
+struct UpToSlab {
+    int smin, smax;
+    bool up;
+    __device__ __forceinline__ bool operator()(int cd) const { return up ? cd <= smax : cd >= smin; }
+};
+
+// ---- pass 2 of one ray as a resumable state -------------------------------------
+
+// The second march of one ray: the same samples as march_v1 (same alpha
+// sequence, same label cache), scattering G_t w_d(p_t) into the cell's 8
+// corner sums, handed to a deposit target whenever the ray leaves a cell
+// (and on request), and accumulating the pose sums S0, S1.  `run` stops
+// before the first sample whose cell fails a predicate on its coordinate
+// along one axis, so a workgroup can march its rays slab by slab.
+template <int M, bool POSE>
+struct AdjRay {
+    RayV1 r;
+    Cell c;
+    float alpha;
+    int t;
+    bool live;
+    float graw[M];
     float acc[8];   // sum over the cell's samples of G w_d: density gradient of corner d
     float acc1[8];  // sum of alpha G w_d (pose only)
-#pragma unroll
-    for (int d = 0; d < 8; d++) {
-        acc[d] = 0.0f;
-        acc1[d] = 0.0f;
+    float hu;       // dL/d(ad_raw) of the material of a cell whose labels agree
+    // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t alpha_t dL/dp_t, in FP64
+    // (thousands of terms of both signs), kept in LDS (S[k * 256] for k < 6,
+    // this thread's column) rather than registers.  Within a cell, dL/dp_t =
+    // G_t grad(rho)(p_t) [+ wt rho_t grad(h)(p_t) if the labels differ, h_d =
+    // dL/d(ad_raw) of corner d's material], and grad(rho) along x is
+    // sum_yz (D[1yz] - D[0yz]) w_yz, so sum_t G_t grad_x(rho) =
+    // sum_yz (D[1yz] - D[0yz]) (acc[0yz] + acc[1yz]) (w_0yz + w_1yz = w_yz):
+    // the G term needs no per-sample gradient.  All of this is linear in the
+    // sums, so a cell may be flushed in parts.
+    double *S;
+
+    __device__ __forceinline__ void set_h() {
+        if (c.uni) hu = get_m<M>(graw, (int)c.m0);
     }
-    float h[8];
-    float hu = 0.0f;  // h of a cell whose labels agree
-    auto set_h = [&]() {
-        if (c.uni) {
-            hu = get_m<M>(graw, (int)c.m0);
-        } else {
+
+    // h_d of the open cell (a cell whose labels differ), recomputed where needed
+    __device__ __forceinline__ void corner_h(float (&h)[8]) const {
 #pragma unroll
-            for (int d = 0; d < 8; d++) h[d] = get_m<M>(graw, (int)corner_label(c.l0, c.l1, d));
+        for (int d = 0; d < 8; d++) h[d] = get_m<M>(graw, (int)corner_label(c.l0, c.l1, d));
+    }
+
+    // Ray setup and the steps before the volume (skip, then the window that
+    // re-establishes the label cache), exactly as march_v1.
+    __device__ __forceinline__ void begin(const Geo1 &g, const VolDev &vol, int udx, int vdx, float step,
+                                          const float (&gr)[M]) {
+#pragma unroll
+        for (int m = 0; m < M; m++) graw[m] = gr[m];
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            acc[d] = 0.0f;
+            acc1[d] = 0.0f;
         }
-    };
-    if (c.have) set_h();
-    // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t alpha_t dL/dp_t: FP32 within a
-    // cell, FP64 across cells (thousands of terms of both signs).  Within a
-    // cell, dL/dp_t = G_t grad(rho)(p_t) [+ wt rho_t grad(h)(p_t) if the labels
-    // differ], and grad(rho) along x is sum_yz (D[1yz] - D[0yz]) w_yz, so
-    // sum_t G_t grad_x(rho) = sum_yz (D[1yz] - D[0yz]) (acc[0yz] + acc[1yz])
-    // (w_0yz + w_1yz = w_yz): the G term needs no per-sample gradient.
-    double S0[3] = {0.0, 0.0, 0.0}, S1[3] = {0.0, 0.0, 0.0};
-    float c0[3] = {0.f, 0.f, 0.f}, c1[3] = {0.f, 0.f, 0.f};  // label-gradient term of mixed cells
-    auto flush_pose = [&]() {
+        hu = 0.0f;
+        if (POSE)
+#pragma unroll
+            for (int k = 0; k < 6; k++) S[k * 256] = 0.0;
+        march_setup_v1(g, udx, vdx, step, r);
+        live = r.trace;
+        alpha = r.minA;
+        t = 0;
+        if (!live) return;
+        const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
+        const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
+        const float rdom = fmaxf(fabsf(rix), fmaxf(fabsf(riy), fabsf(riz)));
+        const float lim = r.mnv - (1.0f / rdom + 2.0f * step);
+        skip_steps(r.N, lim, step, alpha, t);
+        while (t < r.N && alpha < r.mnv) {
+            const float px = sx + alpha * rix - 1.0f;
+            const float py = sy + alpha * riy - 1.0f;
+            const float pz = sz + alpha * riz - 1.0f;
+            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) c.load(vol, px, py, pz, bx, by, bz);
+            alpha += step;
+            ++t;
+        }
+        if (c.have) set_h();
+    }
+
+    __device__ __forceinline__ void flush_pose() {
         const float D[8] = {c.d0.x, c.d0.y, c.d0.z, c.d0.w, c.d1.x, c.d1.y, c.d1.z, c.d1.w};
         float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -119,71 +211,109 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            S0[k] += (double)g0[k] + (double)c0[k];
-            S1[k] += (double)g1[k] + (double)c1[k];
-            c0[k] = 0.0f;
-            c1[k] = 0.0f;
+            S[k * 256] += (double)g0[k];
+            S[(k + 3) * 256] += (double)g1[k];
         }
 #pragma unroll
         for (int d = 0; d < 8; d++) acc1[d] = 0.0f;
-    };
-    while (t < N && alpha <= r.mxv) {
-        const float px = sx + alpha * rix - 1.0f;
-        const float py = sy + alpha * riy - 1.0f;
-        const float pz = sz + alpha * riz - 1.0f;
-        const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-        if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
-            if (c.have) {
-                if (POSE) flush_pose();
-                flush_cell(vol, gD, c.bx, c.by, c.bz, acc);
+    }
+
+    // Hand the open cell's sums to `dep` (the cell stays cached).
+    template <class Dep>
+    __device__ __forceinline__ void flush(const VolDev &vol, Dep &dep) {
+        if (!c.have) return;
+        if (POSE) flush_pose();
+        dep(vol, c.bx, c.by, c.bz, acc);
+    }
+
+    // March on while the cell coordinate along `axis` (clamped, in [-1, n-1])
+    // of each newly entered cell satisfies go(coordinate).  The open cell's
+    // sums are flushed when the ray leaves it or ends.
+    template <class Dep, class Go>
+    __device__ __forceinline__ void run(const Geo1 &g, const VolDev &vol, float step, int axis, Dep &dep, Go go) {
+        if (!live) return;
+        const int n_axis = axis == 0 ? vol.nx : (axis == 1 ? vol.ny : vol.nz);
+        const int N = r.N;
+        const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
+        const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
+        // The sample position is computed one step ahead so that the slab test
+        // sits at the top of the loop (one loop exit, before any state
+        // changes, keeps the state in registers across the caller's slab loop).
+        float px = sx + alpha * rix - 1.0f, py = sy + alpha * riy - 1.0f, pz = sz + alpha * riz - 1.0f;
+        float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+        // one branch-free continuation test, evaluated at the bottom of the loop
+        auto cont_test = [&]() __attribute__((always_inline)) {
+            const bool in_range = (t < N) & (alpha <= r.mxv);
+            const bool same = c.have & (bx == c.bx) & (by == c.by) & (bz == c.bz);
+            const bool ok = go(clamp_cell(axis == 0 ? bx : (axis == 1 ? by : bz), n_axis));
+            return in_range & (same | ok);
+        };
+        bool cont = cont_test();
+        while (cont) {            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
+                flush(vol, dep);
+                c.load(vol, px, py, pz, bx, by, bz);
+                set_h();
             }
-            c.load(vol, px, py, pz, bx, by, bz);
-            set_h();
-        }
-        const float fx = px - bx, fy = py - by, fz = pz - bz;
-        float w[8];
-        weights8(fx, fy, fz, w);
-        const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
-        float sw;
-        if (c.uni) {
-            const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
-            sw = hu * ssum;
-        } else {
-            sw = 0.0f;
+            const float fx = px - bx, fy = py - by, fz = pz - bz;
+            float w[8];
+            weights8(fx, fy, fz, w);
+            const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
+            float sw;
+            if (c.uni) {
+                const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
+                sw = hu * ssum;
+            } else {
+                float h[8];
+                corner_h(h);
+                sw = 0.0f;
 #pragma unroll
-            for (int d = 0; d < 8; d++) sw = fmaf(w[d], h[d], sw);
-        }
-        const float G = wt * sw;
+                for (int d = 0; d < 8; d++) sw = fmaf(w[d], h[d], sw);
+                if (POSE) {
+                    // label-gradient term of a cell whose labels differ (rare)
+                    const float rho = trilerp(c.d0, c.d1, w);
+                    float gh[3];
+                    grad_trilinear(h, fx, fy, fz, gh);
+                    const float wr = wt * rho;
 #pragma unroll
-        for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
-        if (POSE) {
-            const float aG = alpha * G;
-#pragma unroll
-            for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
-            if (!c.uni) {
-                const float rho = trilerp(c.d0, c.d1, w);
-                float gh[3];
-                grad_trilinear(h, fx, fy, fz, gh);
-                const float wr = wt * rho;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    c0[k] = fmaf(wr, gh[k], c0[k]);
-                    c1[k] = fmaf(alpha * wr, gh[k], c1[k]);
+                    for (int k = 0; k < 3; k++) {
+                        S[k * 256] += (double)(wr * gh[k]);
+                        S[(k + 3) * 256] += (double)(alpha * wr * gh[k]);
+                    }
                 }
             }
+            const float G = wt * sw;
+#pragma unroll
+            for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
+            if (POSE) {
+                const float aG = alpha * G;
+#pragma unroll
+                for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
+            }
+            alpha += step;
+            ++t;
+            px = sx + alpha * rix - 1.0f;
+            py = sy + alpha * riy - 1.0f;
+            pz = sz + alpha * riz - 1.0f;
+            bx = floorf(px);
+            by = floorf(py);
+            bz = floorf(pz);
+            cont = cont_test();
         }
-        alpha += step;
-        ++t;
+        if (!(t < N && alpha <= r.mxv)) {
+            flush(vol, dep);
+            live = false;
+        }
     }
-    if (c.have) {
-        if (POSE) flush_pose();
-        flush_cell(vol, gD, c.bx, c.by, c.bz, acc);
-    }
-    if (POSE) {
+
+    // Pose gradient of the finished ray into gpose.
+    __device__ __forceinline__ void end_pose(const Geo1 &g, const float sw_world[3], const float center[3],
+                                             double (&gpose)[6]) const {
+        if (!r.trace) return;
         // p = A x + b - 1 with x = s_w + alpha r_w; perturbed x' = R(w)^T (x - c - t) + c:
         // dp/dt = -A, dp/dw = A [x - c]_x  ->  dL/dt = -A^T S0,
         // dL/dw = (A^T S0) x (s_w - c) + (A^T S1) x r_w
         const float *A = g.A;
+        const double S0[3] = {S[0], S[256], S[512]}, S1[3] = {S[768], S[1024], S[1280]};
         double V0[3], V1[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -200,16 +330,13 @@ __device__ __forceinline__ void adjoint_march(const Geo1 &g, const VolDev &vol, 
         gpose[4] += (V0[2] * q[0] - V0[0] * q[2]) + (V1[2] * rw[0] - V1[0] * rw[2]);
         gpose[5] += (V0[0] * q[1] - V0[1] * q[0]) + (V1[0] * rw[1] - V1[1] * rw[0]);
     }
-}
+};
 
-// Both passes for one pixel with output gradients gi (intensity), gp (photon_prob).
-template <int M, bool ATTEN, bool POSE>
-__device__ __forceinline__ void adjoint_ray(const Geo1 &g, const VolDev &vol, const SpectrumDev &sp, float step,
-                                            int air, int udx, int vdx, float gi, float gp, const float sw[3],
-                                            const float center[3], float *gD, double (&gpose)[6]) {
-    if (gi == 0.0f && gp == 0.0f) return;
-    float ad[M];
-    march_v1<M, ATTEN>(g, vol, udx, vdx, step, air, ad);
+// dL/d(ad_raw[m]) of one pixel from its area densities ad[m] (g/cm^2) for
+// output gradients gi (intensity) and gp (photon_prob); ad_raw = ad * 10 / step.
+template <int M>
+__device__ __forceinline__ void graw_from_ad(const SpectrumDev &sp, const float (&ad)[M], float step, float gi,
+                                             float gp, float (&graw)[M]) {
     float gad[M];
 #pragma unroll
     for (int m = 0; m < M; m++) gad[m] = 0.0f;
@@ -221,10 +348,35 @@ __device__ __forceinline__ void adjoint_ray(const Geo1 &g, const VolDev &vol, co
 #pragma unroll
         for (int m = 0; m < M; m++) gad[m] -= coef * sp.mu[b * M + m];
     }
-    float graw[M];
 #pragma unroll
     for (int m = 0; m < M; m++) graw[m] = gad[m] * step / 10.0f;
-    adjoint_march<M, POSE>(g, vol, udx, vdx, step, graw, sw, center, gD, gpose);
+}
+
+// Pass 1 of one pixel: the forward march, then graw.
+template <int M, bool ATTEN>
+__device__ __forceinline__ void backward_spectral(const Geo1 &g, const VolDev &vol, const SpectrumDev &sp,
+                                                  float step, int air, int udx, int vdx, float gi, float gp,
+                                                  float (&graw)[M]) {
+    float ad[M];
+    march_v1<M, ATTEN>(g, vol, udx, vdx, step, air, ad);
+    graw_from_ad<M>(sp, ad, step, gi, gp, graw);
+}
+
+// Both passes for one pixel, scattering straight into the gradient volume.
+template <int M, bool ATTEN, bool POSE>
+__device__ __forceinline__ void adjoint_ray(const Geo1 &g, const VolDev &vol, const SpectrumDev &sp, float step,
+                                            int air, int udx, int vdx, float gi, float gp, const float sw[3],
+                                            const float center[3], float *gD, double *pose_slots,
+                                            double (&gpose)[6]) {
+    if (gi == 0.0f && gp == 0.0f) return;
+    float graw[M];
+    backward_spectral<M, ATTEN>(g, vol, sp, step, air, udx, vdx, gi, gp, graw);
+    AdjRay<M, POSE> ray;
+    ray.S = pose_slots;
+    ray.begin(g, vol, udx, vdx, step, graw);
+    GlobalDep dep{gD};
+    ray.run(g, vol, step, 0, dep, AllCells{});
+    if (POSE) ray.end_pose(g, sw, center, gpose);
 }
 
 __device__ __forceinline__ void flush_pose_wave(double (&gpose)[6], float *gPose) {
@@ -241,6 +393,7 @@ __device__ __forceinline__ void flush_pose_wave(double (&gpose)[6], float *gPose
 struct AdjArgs {
     const float *gI;  // [H][W] (or job images at out_offset), may be NULL
     const float *gP;  // may be NULL
+    const float *AD;  // saved forward area densities (M planes per job at M * out_offset), may be NULL
     float c[3];       // pose centre in world
     float *gD;        // [nz][ny][nx]
     float *gPose;     // [6] or NULL
@@ -255,6 +408,7 @@ __device__ __forceinline__ int xcd_remap_adj(int bid, int nb) {
 template <int M, bool ATTEN, bool POSE>
 __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step, int air,
                                                     AdjArgs a, float swx, float swy, float swz, int tiles16_x) {
+    __shared__ double pose_slots[POSE ? 6 * 256 : 1];
     const int lb = xcd_remap_adj(blockIdx.x, gridDim.x);
     const int bx = lb % tiles16_x, by = lb / tiles16_x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -265,20 +419,9 @@ __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, Spectrum
         const int64_t idx = (int64_t)vdx * g.width + udx;
         const float sw[3] = {swx, swy, swz};
         adjoint_ray<M, ATTEN, POSE>(g, vol, sp, step, air, udx, vdx, a.gI ? a.gI[idx] : 0.0f,
-                                    a.gP ? a.gP[idx] : 0.0f, sw, a.c, a.gD, gpose);
+                                    a.gP ? a.gP[idx] : 0.0f, sw, a.c, a.gD, pose_slots + threadIdx.x, gpose);
     }
     if (POSE) flush_pose_wave(gpose, a.gPose);
-}
-
-// Wave sum of the pose gradient into an FP64 accumulator (one atomic per wave).
-__device__ __forceinline__ void flush_pose_wave64(const double (&gpose)[6], double *acc) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        double v = gpose[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0 && v != 0.0) unsafeAtomicAdd(acc + k, v);
-    }
 }
 
 __global__ void k_pose_zero(double *acc) {
@@ -289,21 +432,157 @@ __global__ void k_pose_finish(const double *acc, float *gPose) {
     if (threadIdx.x < 6) gPose[threadIdx.x] += (float)acc[threadIdx.x];
 }
 
+// ---- batched adjoint: workgroup tiles, LDS accumulation slab by slab --------------
+
+// A workgroup takes a 16x16-pixel tile of one view (4 waves of 8x8 rays).
+// Its rays form a narrow beam; the beam is cut into slabs of ADJ_SLAB cells
+// along the axis its central ray advances fastest on, and all rays march one
+// slab at a time.  Corner sums go into an LDS box covering the slab
+// (ADJ_SLAB + 1 voxel planes along that axis, ADJ_BOX voxels across, centred
+// on the central ray) with LDS atomics; after each slab the box is added to
+// the gradient volume once per touched voxel.  Corners outside the box (a
+// beam wider than the box, a ray against the slab order) go to the gradient
+// volume directly, so results do not depend on the geometry, only the
+// number of global atomics does.
+constexpr int ADJ_TILE = 16;
+constexpr int ADJ_SLAB = 8;
+constexpr int ADJ_BOX = 24;
+constexpr int ADJ_BOX_CELLS = (ADJ_SLAB + 1) * ADJ_BOX * ADJ_BOX;
+
+// World ray through the detector point (fu, fv) (pixel-edge coordinates).
+__device__ __forceinline__ void ray_through(const float *W, float fu, float fv, float r[3]) {
+    r[0] = fu * W[0] + fv * W[1] + W[2];
+    r[1] = fu * W[3] + fv * W[4] + W[5];
+    r[2] = fu * W[6] + fv * W[7] + W[8];
+}
+
+// Occupancy targets that fit without (or, for 7-8 materials, with a couple
+// of) register spills: 5 waves per SIMD for the density gradient alone, 4
+// with the pose sums.
+template <int POSE>
+struct AdjOcc {
+    static constexpr int waves = POSE ? 4 : 5;
+};
+
 template <int M, bool ATTEN, bool POSE>
-__global__ __launch_bounds__(256) void k_adjoint_jobs(const JobDev *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE>::waves))) void k_adjoint_jobs(const JobDev *__restrict__ jobs, const int2 *__restrict__ items,
                                                       const int *__restrict__ seg, int *counters,
                                                       const float *__restrict__ sw_jobs, VolDev vol, SpectrumDev sp,
                                                       float step, int air, AdjArgs a, double *pose64) {
-    for_each_tile<8>(jobs, items, seg, counters, [&](int j, const JobDev &J, int udx, int vdx, bool valid) {
-        double gpose[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (valid) {
+    __shared__ float box[ADJ_BOX_CELLS];
+    __shared__ double s_pose[4][6];
+    __shared__ double pose_slots[POSE ? 6 * 256 : 1];
+    for (int i = threadIdx.x; i < ADJ_BOX_CELLS; i += blockDim.x) box[i] = 0.0f;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for_each_block_item(jobs, items, seg, counters, [&](int j, const JobDev &J, int tile) __attribute__((always_inline)) {
+        const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+        int udx, vdx;
+        lane_pixel(tx * 2 + (wave & 1), ty * 2 + (wave >> 1), lane, udx, vdx);
+        float gi = 0.0f, gp = 0.0f;
+        if (udx < J.g.width && vdx < J.g.height) {
             const int64_t o = J.out_offset + (int64_t)vdx * J.g.width + udx;
-            const float sw[3] = {sw_jobs[3 * j], sw_jobs[3 * j + 1], sw_jobs[3 * j + 2]};
-            adjoint_ray<M, ATTEN, POSE>(J.g, vol, sp, step, air, udx, vdx, a.gI ? a.gI[o] : 0.0f,
-                                        a.gP ? a.gP[o] : 0.0f, sw, a.c, a.gD, gpose);
+            gi = a.gI ? a.gI[o] : 0.0f;
+            gp = a.gP ? a.gP[o] : 0.0f;
         }
-        // per tile, so the FP64 sums are not held in registers across tiles
-        if (POSE) flush_pose_wave64(gpose, pose64);
+        AdjRay<M, POSE> ray;
+        ray.S = pose_slots + threadIdx.x;
+        ray.live = false;
+        ray.r.trace = false;
+        if (gi != 0.0f || gp != 0.0f) {
+            float graw[M];
+            if (a.AD) {
+                // the forward pass's area densities: no need to march twice
+                const int64_t npix = (int64_t)J.g.width * J.g.height;
+                const float *ad_p = a.AD + M * J.out_offset + (int64_t)vdx * J.g.width + udx;
+                float ad[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) ad[m] = ad_p[m * npix];
+                graw_from_ad<M>(sp, ad, step, gi, gp, graw);
+            } else {
+                backward_spectral<M, ATTEN>(J.g, vol, sp, step, air, udx, vdx, gi, gp, graw);
+            }
+            ray.begin(J.g, vol, udx, vdx, step, graw);
+        }
+        // The tile's central ray (the same value in every thread) fixes the
+        // slab axis, the slab order and the box centre.
+        float rc[3], rci[3];
+        ray_through(J.g.W, (float)(tx * ADJ_TILE) + 0.5f * ADJ_TILE, (float)(ty * ADJ_TILE) + 0.5f * ADJ_TILE, rc);
+#pragma unroll
+        for (int k = 0; k < 3; k++) rci[k] = J.g.A[4 * k] * rc[0] + J.g.A[4 * k + 1] * rc[1] + J.g.A[4 * k + 2] * rc[2];
+        // (selects, not array indexing by `axis`, so nothing lands in scratch memory)
+        int axis = fabsf(rci[1]) > fabsf(rci[0]) ? 1 : 0;
+        float ra = axis ? rci[1] : rci[0];
+        if (fabsf(rci[2]) > fabsf(ra)) {
+            axis = 2;
+            ra = rci[2];
+        }
+        const bool up = !(ra < 0.0f);
+        const int n_axis = axis == 0 ? vol.nx : (axis == 1 ? vol.ny : vol.nz);
+        const float s_axis = axis == 0 ? J.g.s[0] : (axis == 1 ? J.g.s[1] : J.g.s[2]);
+        const int n_slabs = (n_axis + 1 + ADJ_SLAB - 1) / ADJ_SLAB;  // cells -1 .. n-1
+        for (int k = 0; k < n_slabs; k++) {
+            // cells [smin, smax] along the axis; their corners lie in [smin, smax + 1]
+            const int smin = up ? -1 + k * ADJ_SLAB : n_axis - 1 - (k + 1) * ADJ_SLAB + 1;
+            const int smax = smin + ADJ_SLAB - 1;
+            // box across the beam: centred on the central ray at the slab's middle plane
+            float o[3] = {0.f, 0.f, 0.f};
+            if (ra != 0.0f) {
+                const float mid = 0.5f * (float)(smin + smax + 1);
+                const float al = (mid + 1.0f - s_axis) / ra;
+#pragma unroll
+                for (int q = 0; q < 3; q++) o[q] = J.g.s[q] + al * rci[q] - 1.0f;
+            }
+            int org[3], len[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const float f = __builtin_amdgcn_fmed3f(floorf(o[q]), -1048576.0f, 1048576.0f);
+                org[q] = q == axis ? smin : (int)f - ADJ_BOX / 2 + 1;
+                len[q] = q == axis ? ADJ_SLAB + 1 : ADJ_BOX;
+            }
+            BoxDep dep{box, org[0], org[1], org[2], len[0], len[1], len[2], a.gD};
+            ray.run(J.g, vol, step, axis, dep, UpToSlab{smin, smax, up});
+            if (ray.live) ray.flush(vol, dep);
+            __syncthreads();
+            // add the box to the gradient volume and clear it
+            const float inv_x = 1.0f / (float)len[0], inv_y = 1.0f / (float)len[1];
+            const uint32_t nx = vol.nx, ny = vol.ny;
+            for (int i = threadIdx.x; i < ADJ_BOX_CELLS; i += blockDim.x) {
+                const float v = box[i];
+                if (v != 0.0f) {
+                    const int iyz = (int)(((float)i + 0.5f) * inv_x);
+                    const int ix = i - iyz * len[0];
+                    const int iz = (int)(((float)iyz + 0.5f) * inv_y);
+                    const int iy = iyz - iz * len[1];
+                    const int x = org[0] + ix, y = org[1] + iy, z = org[2] + iz;
+                    if (x >= 0 && x < vol.nx && y >= 0 && y < vol.ny && z >= 0 && z < vol.nz)
+                        unsafeAtomicAdd(a.gD + (((uint32_t)z * ny + (uint32_t)y) * nx + (uint32_t)x), v);
+                    box[i] = 0.0f;
+                }
+            }
+            __syncthreads();
+        }
+        // anything left (none when every ray follows the slab order) goes straight out
+        GlobalDep gdep{a.gD};
+        ray.run(J.g, vol, step, axis, gdep, AllCells{});
+        if (POSE) {
+            double gpose[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            const float sw[3] = {sw_jobs[3 * j], sw_jobs[3 * j + 1], sw_jobs[3 * j + 2]};
+            ray.end_pose(J.g, sw, a.c, gpose);
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                double v = gpose[q];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0) s_pose[wave][q] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 6) {
+                const double v = s_pose[0][threadIdx.x] + s_pose[1][threadIdx.x] + s_pose[2][threadIdx.x] +
+                                 s_pose[3][threadIdx.x];
+                if (v != 0.0) unsafeAtomicAdd(pose64 + threadIdx.x, v);
+            }
+            __syncthreads();
+        }
     });
 }
 
@@ -396,7 +675,7 @@ extern "C" int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t
 extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_job_t *jobs, int n_jobs,
                                 const float *grad_intensity, const float *grad_photon_prob,
                                 const float *source_world, const float *pose_center_world,
-                                float *grad_density, float *grad_pose, void *stream) {
+                                const float *area_density, float *grad_density, float *grad_pose, void *stream) {
     int rc = check_adj(ctx, vol, grad_intensity, grad_photon_prob, grad_density, grad_pose, source_world,
                        pose_center_world);
     if (rc) return rc;
@@ -409,8 +688,9 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
     if (prev != ctx->device) DRR_HIP_CHECK(hipSetDevice(ctx->device));
     const drr_volume *vols[1] = {vol};
     JobTable t;
-    const drr_batch_tuning_t tune{8, 8, 4, 8};
-    rc = drr_jobs_build(ctx, vols, 1, jobs, n_jobs, tune, t);
+    // 16x16-pixel workgroup tiles, 4 waves per workgroup, 4 persistent workgroups per CU
+    const drr_batch_tuning_t tune{ADJ_TILE, ADJ_TILE, 4, 4};
+    rc = drr_jobs_build_tiles(ctx, vols, 1, jobs, n_jobs, tune, t);
     if (rc != DRR_OK) return rc;
     // scratch: FP64 pose accumulator [6], then the job sources [3 * n_jobs]
     double *d_pose64 = nullptr;
@@ -430,6 +710,7 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
     AdjArgs a{};
     a.gI = grad_intensity;
     a.gP = grad_photon_prob;
+    a.AD = area_density;
     for (int i = 0; i < 3; i++) a.c[i] = pose_center_world ? pose_center_world[i] : 0.0f;
     a.gD = grad_density;
     a.gPose = grad_pose;

@@ -37,12 +37,19 @@ void drr_jobs_free(JobTable &t) {
 
 int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                    int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
-    if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
-        return drr_set_error(DRR_ERR_INVALID, "jobs: need >= 1 volume and 1..65535 jobs");
     if (!((tune.tile_w == 8 && tune.tile_h == 8) || (tune.tile_w == 16 && tune.tile_h == 4)))
         return drr_set_error(DRR_ERR_INVALID, "jobs: tile must be 8x8 or 16x4");
     if (tune.waves_per_block < 1 || tune.waves_per_block > 4 || tune.blocks_per_cu < 1 || tune.blocks_per_cu > 32)
         return drr_set_error(DRR_ERR_INVALID, "jobs: waves_per_block in [1,4], blocks_per_cu in [1,32]");
+    return drr_jobs_build_tiles(ctx, vols, n_vols, jobs, n_jobs, tune, t);
+}
+
+int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                         int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
+    if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
+        return drr_set_error(DRR_ERR_INVALID, "jobs: need >= 1 volume and 1..65535 jobs");
+    if (tune.tile_w < 1 || tune.tile_h < 1 || tune.waves_per_block < 1 || tune.blocks_per_cu < 1)
+        return drr_set_error(DRR_ERR_INVALID, "jobs: bad tiling");
     for (int v = 0; v < n_vols; v++)
         if (!vols[v] || vols[v]->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "jobs: NULL or foreign volume");
     t.n_jobs = n_jobs;
@@ -127,40 +134,48 @@ template <int M, bool ATTEN, int TW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M + (ATTEN ? 1 : 0)>::waves)))
 void k_batch_march(const JobDev *__restrict__ jobs, const VolDev *__restrict__ vols,
                    const int2 *__restrict__ items, const int *__restrict__ seg, int *counters,
-                   SpectrumDev sp, float step, int air, float *__restrict__ I, float *__restrict__ P) {
+                   SpectrumDev sp, float step, int air, float *__restrict__ I, float *__restrict__ P,
+                   float *__restrict__ AD) {
     for_each_tile<TW>(jobs, items, seg, counters, [&](int, const JobDev &J, int udx, int vdx, bool valid) {
         if (!valid) return;
         float ad[M];
         march_v1<M, ATTEN>(J.g, vols[J.vol], udx, vdx, step, air, ad);
         float inten, pp;
         spectral<M>(sp, ad, inten, pp);
-        const int64_t o = J.out_offset + (int64_t)vdx * J.g.width + udx;
+        const int64_t pix = (int64_t)vdx * J.g.width + udx;
+        const int64_t o = J.out_offset + pix;
         I[o] = inten;
         if (P) P[o] = pp;
+        if (AD) {
+            // job block of M planes at M * out_offset (jobs' images do not overlap)
+            const int64_t npix = (int64_t)J.g.width * J.g.height;
+#pragma unroll
+            for (int m = 0; m < M; m++) AD[M * J.out_offset + m * npix + pix] = ad[m];
+        }
     });
 }
 
 template <int M, bool ATTEN>
 static void launch_march_tw(int tw, const JobTable &t, hipStream_t st, SpectrumDev sp, float step, int air,
-                            float *I, float *P) {
+                            float *I, float *P, float *AD) {
     dim3 g(t.grid), b(t.block);
     if (tw == 16)
         k_batch_march<M, ATTEN, 16><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
-                                                      step, air, I, P);
+                                                      step, air, I, P, AD);
     else
         k_batch_march<M, ATTEN, 8><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
-                                                     step, air, I, P);
+                                                     step, air, I, P, AD);
 }
 
 static void launch_march(const JobTable &t, int M, bool atten, int tw, hipStream_t st, SpectrumDev sp,
-                         float step, int air, float *I, float *P) {
+                         float step, int air, float *I, float *P, float *AD) {
     switch (M) {
 #define DRR_CASE(MM)                                                             \
     case MM:                                                                     \
         if (atten)                                                               \
-            launch_march_tw<MM, true>(tw, t, st, sp, step, air, I, P);           \
+            launch_march_tw<MM, true>(tw, t, st, sp, step, air, I, P, AD);       \
         else                                                                     \
-            launch_march_tw<MM, false>(tw, t, st, sp, step, air, I, P);          \
+            launch_march_tw<MM, false>(tw, t, st, sp, step, air, I, P, AD);      \
         break;
         DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
 #undef DRR_CASE
@@ -190,7 +205,7 @@ static hipError_t capture(drr_batch *B, SpectrumDev sp, float step, int air, flo
     hipError_t e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     k_jobs_reset<<<1, 64, 0, B->cap>>>(B->t.d_counters);
-    launch_march(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P);
+    launch_march(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P, nullptr);
     e = hipStreamEndCapture(B->cap, &B->graph_march);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&B->exec_march, B->graph_march, nullptr, nullptr, 0);
@@ -291,7 +306,8 @@ extern "C" int drr_batch_destroy(drr_batch *B) {
 }
 
 extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
-                                int n_jobs, float *intensity, float *photon_prob, void *stream) {
+                                int n_jobs, float *intensity, float *photon_prob, float *area_density,
+                                void *stream) {
     if (!ctx || !vols || !jobs || !intensity)
         return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs: NULL argument");
     if (!ctx->have_scene || ctx->n_bins < 1)
@@ -306,7 +322,7 @@ extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);
     launch_march(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
-                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob);
+                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
     hipError_t e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;

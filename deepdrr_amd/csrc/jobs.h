@@ -36,8 +36,12 @@ struct JobTable {
 };
 
 // Build (and upload) the job table; host-side validation included.
+// drr_jobs_build accepts the wave tiles of the batched march (8x8, 16x4);
+// drr_jobs_build_tiles any tile (the adjoint's 16x16 workgroup tiles).
 int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                    int n_jobs, const drr_batch_tuning_t &tune, JobTable &t);
+int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                         int n_jobs, const drr_batch_tuning_t &tune, JobTable &t);
 void drr_jobs_free(JobTable &t);
 
 __global__ void k_jobs_reset(int *counters);
@@ -69,5 +73,45 @@ __device__ __forceinline__ void for_each_tile(const JobDev *__restrict__ jobs, c
         const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
         const int udx = tx * TW + (lane % TW), vdx = ty * TH + (lane / TW);
         body(j, J, udx, vdx, udx < J.g.width && vdx < J.g.height);
+    }
+}
+
+// Persistent loop over workgroup-sized items: thread 0 takes the next item
+// (same per-XCD queues and stealing as for_each_tile) and the whole block
+// receives it through LDS, so every thread sees the same items and the same
+// number of iterations (the body may use __syncthreads).  body(j, J, tile).
+template <typename Body>
+__device__ __forceinline__ void for_each_block_item(const JobDev *__restrict__ jobs, const int2 *__restrict__ items,
+                                                    const int *__restrict__ seg, int *counters, Body body) {
+    __shared__ int2 s_item;
+    __shared__ int s_q, s_exhausted;
+    if (threadIdx.x == 0) {
+        s_q = blockIdx.x & 7;
+        s_exhausted = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int2 it = make_int2(-1, -1);
+            while (s_exhausted < 8) {
+                const int q = s_q;
+                const int idx = seg[q] + atomicAdd(&counters[q], 1);
+                if (idx < seg[q + 1]) {
+                    it = items[idx];
+                    break;
+                }
+                s_q = (q + 1) & 7;
+                ++s_exhausted;
+            }
+            s_item = it;
+        }
+        __syncthreads();
+        // the same value in every lane: make it scalar so the job's uniforms
+        // are loaded once per wave into scalar registers
+        const int j = __builtin_amdgcn_readfirstlane(s_item.x);
+        const int tile = __builtin_amdgcn_readfirstlane(s_item.y);
+        __syncthreads();  // s_item is rewritten only after every thread has read it
+        if (j < 0) break;
+        body(j, jobs[j], tile);
     }
 }

@@ -161,9 +161,11 @@ int drr_batch_destroy(drr_batch *batch);
 /* One-shot forms of the batch (no graph, job tables built and freed per call;
  * the call returns after the launch completes).  drr_project_jobs writes each
  * job's intensity / photon_prob image at its out_offset (photon_prob may be
- * NULL). */
+ * NULL).  area_density (optional, may be NULL) receives each job's
+ * per-material area densities (g/cm^2) as n_materials planes of [height][width]
+ * starting at n_materials * out_offset -- what drr_adjoint_jobs can reuse. */
 int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
-                     int n_jobs, float *intensity, float *photon_prob, void *stream);
+                     int n_jobs, float *intensity, float *photon_prob, float *area_density, void *stream);
 
 /* ---- device epilogue --------------------------------------------------------- */
 
@@ -202,11 +204,13 @@ int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t *view,
 
 /* drr_adjoint for many views of one volume in one persistent launch (jobs as
  * in drr_batch_create, all with volume_index 0; the gradient images are read
- * at each job's out_offset).  source_world: 3 floats per job (host). */
+ * at each job's out_offset).  source_world: 3 floats per job (host).
+ * area_density: NULL, or the area densities drr_project_jobs wrote for the
+ * same jobs, volume and scene, which saves re-marching the forward pass. */
 int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_job_t *jobs, int n_jobs,
                      const float *grad_intensity, const float *grad_photon_prob,
                      const float *source_world, const float *pose_center_world,
-                     float *grad_density, float *grad_pose, void *stream);
+                     const float *area_density, float *grad_density, float *grad_pose, void *stream);
 
 /* ---- Monte-Carlo scatter ----------------------------------------------------- */
 

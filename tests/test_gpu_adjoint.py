@@ -225,8 +225,9 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
             jobs[i].out_offset = i * H * Wd
         I_j = torch.empty((len(views), H, Wd), device=gpu)
         P_j = torch.empty_like(I_j)
+        AD_j = torch.empty((len(views), len(MATS), H, Wd), device=gpu)
         _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I_j), _abi.ptr(P_j),
-                  _abi.stream_ptr())
+                  _abi.ptr(AD_j), _abi.stream_ptr())
         rng = np.random.default_rng(2)
         gI = torch.tensor(rng.standard_normal((len(views), H, Wd)).astype(np.float32), device=gpu)
         gPp = torch.tensor(rng.standard_normal((len(views), H, Wd)).astype(np.float32), device=gpu)
@@ -235,16 +236,24 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
         gD_j = torch.zeros_like(d_kji)
         gT_j = torch.zeros(6, device=gpu)
         _abi.call("drr_adjoint_jobs", dp.ctx, h, jobs, len(views), _abi.ptr(gI), _abi.ptr(gPp),
-                  src.ctypes.data, c.ctypes.data, _abi.ptr(gD_j), _abi.ptr(gT_j), _abi.stream_ptr())
+                  src.ctypes.data, c.ctypes.data, None, _abi.ptr(gD_j), _abi.ptr(gT_j), _abi.stream_ptr())
+        # reusing the forward's area densities gives the same gradients
+        gD_r = torch.zeros_like(d_kji)
+        gT_r = torch.zeros(6, device=gpu)
+        _abi.call("drr_adjoint_jobs", dp.ctx, h, jobs, len(views), _abi.ptr(gI), _abi.ptr(gPp),
+                  src.ctypes.data, c.ctypes.data, _abi.ptr(AD_j), _abi.ptr(gD_r), _abi.ptr(gT_r),
+                  _abi.stream_ptr())
         gD_s = torch.zeros_like(d_kji)
         gT_s = torch.zeros(6, device=gpu)
         for i, (v, s) in enumerate(views):
             I = torch.empty((H, Wd), device=gpu)
             Pp = torch.empty_like(I)
-            _abi.call("drr_project", dp.ctx, varr, C.byref(v), _abi.ptr(I), _abi.ptr(Pp), None, None,
+            AD = torch.empty((len(MATS), H, Wd), device=gpu)
+            _abi.call("drr_project", dp.ctx, varr, C.byref(v), _abi.ptr(I), _abi.ptr(Pp), _abi.ptr(AD), None,
                       _abi.stream_ptr())
             torch.cuda.synchronize()
             assert torch.equal(I, I_j[i]) and torch.equal(Pp, P_j[i]), f"view {i}"
+            assert torch.equal(AD, AD_j[i]), f"area densities of view {i}"
             _abi.call("drr_adjoint", dp.ctx, h, C.byref(v), _abi.ptr(gI[i]), _abi.ptr(gPp[i]),
                       s.ctypes.data, c.ctypes.data, _abi.ptr(gD_s), _abi.ptr(gT_s), _abi.stream_ptr())
         torch.cuda.synchronize()
@@ -252,6 +261,10 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
         a, b = gD_j.cpu().numpy(), gD_s.cpu().numpy()
         assert np.abs(a).max() > 0
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+        r_ = gD_r.cpu().numpy()
+        assert np.linalg.norm(r_ - a) <= 1e-6 * np.linalg.norm(a)
+        np.testing.assert_allclose(gT_r.cpu().numpy(), gT_j.cpu().numpy(), rtol=1e-5,
+                                   atol=1e-6 * float(np.abs(gT_j.cpu().numpy()).max()))
         np.testing.assert_allclose(gT_j.cpu().numpy(), gT_s.cpu().numpy(), rtol=1e-4,
                                    atol=1e-5 * float(np.abs(gT_s.cpu().numpy()).max()))
     finally:
