@@ -167,9 +167,11 @@ def measure_mixed(torch, _abi, ctx, clinical_handle, geom_clinical, step, B=256)
                 detectors="384^2/768^2/1536^2", volumes="512x512x400, 256x256x300, 128^3")
 
 
-def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1):
-    """Forward and backward (density + pose) of 64 views 256^2 of a 192^3 volume."""
-    from deepdrr_amd import phantoms
+def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1, repeats=3):
+    """Forward and backward of 64 views 256^2 of a 192^3 density: MSE on the
+    intensity against a target rendered at a perturbed pose (+-5 mm, +-3 deg,
+    seed 1); backward for density and 6-DoF pose, and for density alone."""
+    from deepdrr_amd import _abi, phantoms
     from deepdrr_amd.autograd import DifferentiableProjector
     from deepdrr_amd.spectral_data import synthetic_120kv
 
@@ -183,32 +185,54 @@ def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1):
     ang = np.linspace(0, 2 * np.pi, n_views, endpoint=False)
     projs = [phantoms.pose_projection(det, det, 1.6, [0, 0, 0], [np.sin(a), np.cos(a), 0.0], 0.5)
              for a in ang]
+    rng = np.random.default_rng(1)
+    pert = np.concatenate([rng.uniform(-5, 5, 3), np.deg2rad(rng.uniform(-3, 3, 3))]).astype(np.float32)
     dens = torch.tensor(vol.data, device="cuda", requires_grad=True)
-    theta = torch.zeros(6, device="cuda", requires_grad=True)
-    target = None
-    for it in range(2):  # warm-up, then timed
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        I = dp(dens, theta, projs)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        if target is None:
-            target = (I.detach() * 0.98)
-        loss = ((I - target) ** 2).mean()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        loss.backward()
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        dens.grad = None
-        theta.grad = None
+    with torch.no_grad():
+        target = dp(dens.detach(), torch.tensor(pert, device="cuda"), projs)
+
+    def kernel_ms():
+        ms = C.c_float()
+        _abi.call("drr_ctx_last_kernel_ms", dp.ctx, C.byref(ms))
+        return ms.value
+
+    out = {}
+    for mode in ("pose", "density"):
+        theta = torch.zeros(6, device="cuda", requires_grad=(mode == "pose"))
+        fw, bw, fk, bk = [], [], [], []
+        for it in range(repeats + 1):  # first one warms up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            I = dp(dens, theta, projs)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            kf = kernel_ms()
+            loss = ((I - target) ** 2).mean()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            kb = kernel_ms()
+            dens.grad = None
+            if it:
+                fw.append(t1 - t0)
+                bw.append(t3 - t2)
+                fk.append(kf)
+                bk.append(kb)
+        out[mode] = (float(np.median(fw)) * 1e3 / n_views, float(np.median(bw)) * 1e3 / n_views,
+                     float(np.median(fk)) / n_views, float(np.median(bk)) / n_views)
     dp.close()
-    fwd = (t1 - t0) * 1e3 / n_views
-    bwd = (t3 - t2) * 1e3 / n_views
+    fwd, bwd, fk, bk = out["pose"]
+    _, bwd_d, _, bk_d = out["density"]
     return dict(volume=f"{n}^3", views=n_views, detector=f"{det}^2", forward_ms_per_view=fwd,
                 adjoint_ms_per_view=bwd, adjoint_over_forward=bwd / fwd,
-                note="wall time of one drr_project_jobs / one drr_adjoint_jobs launch over all views "
-                     "(density and 6-DoF pose gradients)")
+                adjoint_density_only_ms_per_view=bwd_d, adjoint_density_only_over_forward=bwd_d / fwd,
+                forward_kernel_ms_per_view=fk, adjoint_kernel_ms_per_view=bk,
+                adjoint_density_only_kernel_ms_per_view=bk_d,
+                note="wall time (median of 3) of one drr_project_jobs launch (forward, saving area "
+                     "densities) and one drr_adjoint_jobs launch (backward) over all 64 views; kernel_ms "
+                     "from HIP events around the launches; MSE against a target at a perturbed pose")
 
 
 def measure_mc(torch, _abi, ctx, handle, geom, n_hist):
@@ -376,7 +400,9 @@ def main():
     if not args.quick:
         log("adjoint (192^3, 64 views of 256^2)")
         result["adjoint"] = measure_adjoint(torch)
-        log(f"adjoint: {result['adjoint']['adjoint_ms_per_view']:.2f} ms/view")
+        a = result["adjoint"]
+        log(f"adjoint: {a['adjoint_ms_per_view']:.3f} ms/view ({a['adjoint_over_forward']:.2f}x forward), "
+            f"density only {a['adjoint_density_only_ms_per_view']:.3f} ms/view")
     print(json.dumps(result), flush=True)
 
 

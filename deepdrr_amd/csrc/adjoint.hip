@@ -107,15 +107,20 @@ struct BoxDep {
     }
 };
 
-// Which cells a call of AdjRay::run may enter: all, or those up to the end of
-// the current slab (along the slab axis, in march order).
+// Which cells a call of AdjRay::run may enter, given the floor b of the
+// sample position along the slab axis: all, or those up to the end of the
+// current slab in march order (sgn * b <= thr; the slab's cell range
+// [smin, smax] after clamping to [-1, n-1], folded into one threshold).
 struct AllCells {
-    __device__ __forceinline__ bool operator()(int) const { return true; }
+    __device__ __forceinline__ bool operator()(float) const { return true; }
 };
 struct UpToSlab {
-    int smin, smax;
-    bool up;
-    __device__ __forceinline__ bool operator()(int cd) const { return up ? cd <= smax : cd >= smin; }
+    float sgn, thr;
+    __device__ __forceinline__ UpToSlab(int smin, int smax, bool up, int n_axis) {
+        sgn = up ? 1.0f : -1.0f;
+        thr = up ? (smax >= n_axis - 1 ? INFINITY : (float)smax) : (smin <= -1 ? INFINITY : -(float)smin);
+    }
+    __device__ __forceinline__ bool operator()(float b) const { return sgn * b <= thr; }
 };
 
 // ---- pass 2 of one ray as a resumable state -------------------------------------
@@ -226,13 +231,12 @@ struct AdjRay {
         dep(vol, c.bx, c.by, c.bz, acc);
     }
 
-    // March on while the cell coordinate along `axis` (clamped, in [-1, n-1])
-    // of each newly entered cell satisfies go(coordinate).  The open cell's
+    // March on while the floor of the sample position along `axis` of each
+    // newly entered cell satisfies go(floor).  The open cell's
     // sums are flushed when the ray leaves it or ends.
     template <class Dep, class Go>
     __device__ __forceinline__ void run(const Geo1 &g, const VolDev &vol, float step, int axis, Dep &dep, Go go) {
         if (!live) return;
-        const int n_axis = axis == 0 ? vol.nx : (axis == 1 ? vol.ny : vol.nz);
         const int N = r.N;
         const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
         const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
@@ -245,7 +249,7 @@ struct AdjRay {
         auto cont_test = [&]() __attribute__((always_inline)) {
             const bool in_range = (t < N) & (alpha <= r.mxv);
             const bool same = c.have & (bx == c.bx) & (by == c.by) & (bz == c.bz);
-            const bool ok = go(clamp_cell(axis == 0 ? bx : (axis == 1 ? by : bz), n_axis));
+            const bool ok = go(axis == 0 ? bx : (axis == 1 ? by : bz));
             return in_range & (same | ok);
         };
         bool cont = cont_test();
@@ -456,12 +460,12 @@ __device__ __forceinline__ void ray_through(const float *W, float fu, float fv, 
     r[2] = fu * W[6] + fv * W[7] + W[8];
 }
 
-// Occupancy targets that fit without (or, for 7-8 materials, with a couple
-// of) register spills: 5 waves per SIMD for the density gradient alone, 4
-// with the pose sums.
+// Occupancy targets that fit without (or, for more materials, with a few)
+// register spills: 6 waves per SIMD for the density gradient alone, 4 with
+// the pose sums.
 template <int POSE>
 struct AdjOcc {
-    static constexpr int waves = POSE ? 4 : 5;
+    static constexpr int waves = POSE ? 4 : 6;
 };
 
 template <int M, bool ATTEN, bool POSE>
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE
                 len[q] = q == axis ? ADJ_SLAB + 1 : ADJ_BOX;
             }
             BoxDep dep{box, org[0], org[1], org[2], len[0], len[1], len[2], a.gD};
-            ray.run(J.g, vol, step, axis, dep, UpToSlab{smin, smax, up});
+            ray.run(J.g, vol, step, axis, dep, UpToSlab(smin, smax, up, n_axis));
             if (ray.live) ray.flush(vol, dep);
             __syncthreads();
             // add the box to the gradient volume and clear it
@@ -688,8 +692,10 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
     if (prev != ctx->device) DRR_HIP_CHECK(hipSetDevice(ctx->device));
     const drr_volume *vols[1] = {vol};
     JobTable t;
-    // 16x16-pixel workgroup tiles, 4 waves per workgroup, 4 persistent workgroups per CU
-    const drr_batch_tuning_t tune{ADJ_TILE, ADJ_TILE, 4, 4};
+    // 16x16-pixel workgroup tiles, 4 waves per workgroup; as many persistent
+    // workgroups per CU as the kernel's occupancy target holds (LDS allows it)
+    const bool pose = grad_pose != nullptr;
+    const drr_batch_tuning_t tune{ADJ_TILE, ADJ_TILE, 4, pose ? AdjOcc<1>::waves : AdjOcc<0>::waves};
     rc = drr_jobs_build_tiles(ctx, vols, 1, jobs, n_jobs, tune, t);
     if (rc != DRR_OK) return rc;
     // scratch: FP64 pose accumulator [6], then the job sources [3 * n_jobs]
@@ -716,7 +722,6 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
     a.gPose = grad_pose;
     const drr_scene_t &sc = ctx->scene;
     hipStream_t st = (hipStream_t)stream;
-    const bool pose = grad_pose != nullptr;
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);
     if (pose) k_pose_zero<<<1, 64, 0, st>>>(d_pose64);
