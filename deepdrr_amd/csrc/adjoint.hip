@@ -54,6 +54,13 @@ struct CellCorners {
 // Straight to the gradient volume with float atomics.
 struct GlobalDep {
     float *gD;
+    // some of the 8 corners (the others are zero in v and skipped per lane)
+    __device__ __forceinline__ void part(const VolDev &vol, float bx, float by, float bz, const float (&v)[8]) {
+        float a[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) a[d] = v[d];
+        (*this)(vol, bx, by, bz, a);
+    }
     __device__ __forceinline__ void operator()(const VolDev &vol, float bx, float by, float bz, float (&acc)[8]) {
         const CellCorners k(vol, bx, by, bz);
         const uint32_t nx = vol.nx, ny = vol.ny;
@@ -73,7 +80,19 @@ struct BoxDep {
     float *box;
     int ox, oy, oz, Lx, Ly, Lz;
     float *gD;
+    // some of the 8 corners: those with a zero sum are skipped lane by lane
+    // (an LDS atomic costs per active lane)
+    __device__ __forceinline__ void part(const VolDev &vol, float bx, float by, float bz, const float (&v)[8]) {
+        float a[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) a[d] = v[d];
+        deposit<true>(vol, bx, by, bz, a);
+    }
     __device__ __forceinline__ void operator()(const VolDev &vol, float bx, float by, float bz, float (&acc)[8]) {
+        deposit<false>(vol, bx, by, bz, acc);
+    }
+    template <bool SKIP_ZERO>
+    __device__ __forceinline__ void deposit(const VolDev &vol, float bx, float by, float bz, float (&acc)[8]) {
         const CellCorners k(vol, bx, by, bz);
         const int ix[2] = {(int)k.x0 - ox, (int)k.x1 - ox}, iy[2] = {(int)k.y0 - oy, (int)k.y1 - oy},
                   iz[2] = {(int)k.z0 - oz, (int)k.z1 - oz};
@@ -85,7 +104,8 @@ struct BoxDep {
             const int row[4] = {(iz[0] * Ly + iy[0]) * Lx, (iz[0] * Ly + iy[1]) * Lx, (iz[1] * Ly + iy[0]) * Lx,
                                 (iz[1] * Ly + iy[1]) * Lx};
 #pragma unroll
-            for (int d = 0; d < 8; d++) atomicAdd(box + (row[d >> 1] + ix[d & 1]), acc[d]);
+            for (int d = 0; d < 8; d++)
+                if (!SKIP_ZERO || acc[d] != 0.0f) atomicAdd(box + (row[d >> 1] + ix[d & 1]), acc[d]);
         } else {
             const uint32_t nx = vol.nx, ny = vol.ny;
 #pragma unroll
@@ -150,8 +170,14 @@ struct AdjRay {
     // sum_yz (D[1yz] - D[0yz]) w_yz, so sum_t G_t grad_x(rho) =
     // sum_yz (D[1yz] - D[0yz]) (acc[0yz] + acc[1yz]) (w_0yz + w_1yz = w_yz):
     // the G term needs no per-sample gradient.  All of this is linear in the
-    // sums, so a cell may be flushed in parts.
+    // sums, so a cell may be flushed in parts.  Each thread owns its slots, so
+    // the LDS atomic adds below never contend: they are used because they
+    // need no read back (no wait on the LDS round trip).
     double *S;
+
+    __device__ __forceinline__ void add_S(int k, double v) {
+        __hip_atomic_fetch_add(S + k * 256, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
 
     __device__ __forceinline__ void set_h() {
         if (c.uni) hu = get_m<M>(graw, (int)c.m0);
@@ -216,8 +242,8 @@ struct AdjRay {
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            S[k * 256] += (double)g0[k];
-            S[(k + 3) * 256] += (double)g1[k];
+            add_S(k, (double)g0[k]);
+            add_S(k + 3, (double)g1[k]);
         }
 #pragma unroll
         for (int d = 0; d < 8; d++) acc1[d] = 0.0f;
@@ -229,6 +255,52 @@ struct AdjRay {
         if (!c.have) return;
         if (POSE) flush_pose();
         dep(vol, c.bx, c.by, c.bz, acc);
+    }
+
+    // The ray leaves the open cell through its face normal to axis k (s = +1
+    // towards +k, -1 towards -k), into the neighbouring cell.  The 4 corners
+    // on that face are corners of the next cell too: their sums stay in
+    // registers (moved to the next cell's corner slots) and only the 4
+    // corners left behind are handed to `dep`.  The open cell's pose term
+    // includes the sums carried in from the previous cell; carry_pose()
+    // removes their share when they are carried into a new cell.
+    template <class Dep>
+    __device__ __forceinline__ void leave_through_face(const VolDev &vol, Dep &dep, int k, int s) {
+        if (POSE) flush_pose();
+        const int f = s < 0 ? 1 : 0;  // bit along k of the corners left behind
+        // (masks and products instead of selects between array elements, which
+        // the compiler would turn into indexed loads from scratch memory)
+        const float m0 = k == 0 ? 1.0f : 0.0f, m1 = k == 1 ? 1.0f : 0.0f, m2 = k == 2 ? 1.0f : 0.0f;
+        float leave[8], nacc[8];
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const int bit = k == 0 ? (d & 1) : (k == 1 ? ((d >> 1) & 1) : (d >> 2));
+            const float on = bit == f ? 1.0f : 0.0f;
+            leave[d] = on * acc[d];
+            // next cell's corner d is this cell's corner d ^ (1 << k) when on the shared face
+            nacc[d] = on * (m0 * acc[d ^ 1] + m1 * acc[d ^ 2] + m2 * acc[d ^ 4]);
+        }
+        dep.part(vol, c.bx, c.by, c.bz, leave);
+#pragma unroll
+        for (int d = 0; d < 8; d++) acc[d] = nacc[d];
+    }
+
+    // After the next cell is loaded: subtract the pose term of the sums just
+    // carried into it (flush_pose will count them with this cell's densities).
+    __device__ __forceinline__ void carry_pose() {
+        const float D[8] = {c.d0.x, c.d0.y, c.d0.z, c.d0.w, c.d1.x, c.d1.y, c.d1.z, c.d1.w};
+        float g0[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int b = 1 << k;
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                if (d & b) continue;
+                g0[k] = fmaf(D[d | b] - D[d], acc[d] + acc[d | b], g0[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) add_S(k, -(double)g0[k]);
     }
 
     // March on while the floor of the sample position along `axis` of each
@@ -254,8 +326,23 @@ struct AdjRay {
         };
         bool cont = cont_test();
         while (cont) {            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
-                flush(vol, dep);
+                // one step to a face neighbour (in clamped cell coordinates): carry
+                // the shared face; anything else: hand over all 8 corners
+                int dd[3] = {0, 0, 0};
+                if (c.have) {
+                    dd[0] = clamp_cell(bx, vol.nx) - clamp_cell(c.bx, vol.nx);
+                    dd[1] = clamp_cell(by, vol.ny) - clamp_cell(c.by, vol.ny);
+                    dd[2] = clamp_cell(bz, vol.nz) - clamp_cell(c.bz, vol.nz);
+                }
+                const bool face = (abs(dd[0]) + abs(dd[1]) + abs(dd[2])) == 1;
+                if (face) {
+                    const int k = dd[0] ? 0 : (dd[1] ? 1 : 2);
+                    leave_through_face(vol, dep, k, dd[0] + dd[1] + dd[2]);
+                } else {
+                    flush(vol, dep);
+                }
                 c.load(vol, px, py, pz, bx, by, bz);
+                if (POSE && face) carry_pose();
                 set_h();
             }
             const float fx = px - bx, fy = py - by, fz = pz - bz;
@@ -280,8 +367,8 @@ struct AdjRay {
                     const float wr = wt * rho;
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        S[k * 256] += (double)(wr * gh[k]);
-                        S[(k + 3) * 256] += (double)(alpha * wr * gh[k]);
+                        add_S(k, (double)(wr * gh[k]));
+                        add_S(k + 3, (double)(alpha * wr * gh[k]));
                     }
                 }
             }
@@ -461,11 +548,11 @@ __device__ __forceinline__ void ray_through(const float *W, float fu, float fv, 
 }
 
 // Occupancy targets that fit without (or, for more materials, with a few)
-// register spills: 6 waves per SIMD for the density gradient alone, 4 with
+// register spills: 5 waves per SIMD for the density gradient alone, 4 with
 // the pose sums.
 template <int POSE>
 struct AdjOcc {
-    static constexpr int waves = POSE ? 4 : 6;
+    static constexpr int waves = POSE ? 4 : 5;
 };
 
 template <int M, bool ATTEN, bool POSE>
