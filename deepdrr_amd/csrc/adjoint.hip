@@ -7,14 +7,17 @@
 // semantics): for every ray
 //   ad[m]  = step/10 * sum_t wt_t rho(p_t) seg_m(p_t)
 //   I      = sum_b E_b pdf_b exp(-sum_m ad[m] mu[b][m]),  P likewise without E_b
-// pass 1 re-runs the forward march for ad[], pass 2 re-marches the same
-// samples and scatters
+// pass 1 re-runs the forward march for ad[] (or reads the area densities the
+// forward pass saved), pass 2 re-marches the same samples and scatters
 //   dL/dD_corner += wt_t * (sum_d w_d h_d) * w_corner,   h_d = dL/dad_raw[label_d]
-// with one atomic flush per cell change (per-corner sums are kept in
-// registers while the ray stays in the cell), and accumulates the pose
-// gradient from dL/dp_t = wt_t [ (sum_d w_d h_d) grad(rho) + rho grad(sum_d h_d w_d) ]
+// with per-corner sums kept in registers while the ray stays in a cell and
+// handed over when it leaves, and accumulates the pose gradient from
+//   dL/dp_t = wt_t [ (sum_d w_d h_d) grad(rho) + rho grad(sum_d h_d w_d) ]
 // (the second term vanishes in a cell whose labels agree).  Nothing is stored
-// per sample.  drr_adjoint_jobs runs many views in one persistent launch.
+// per sample.  drr_adjoint scatters with global float atomics; drr_adjoint_jobs
+// runs many views in one persistent launch whose workgroups gather the
+// scatter of their 16x16-ray tile in LDS, slab by slab along the beam, and
+// add it to the gradient volume once per touched voxel.
 #include <hip/hip_runtime.h>
 
 #include "drr_internal.h"
@@ -257,16 +260,16 @@ struct AdjRay {
         dep(vol, c.bx, c.by, c.bz, acc);
     }
 
-    // The ray leaves the open cell through its face normal to axis k (s = +1
-    // towards +k, -1 towards -k), into the neighbouring cell.  The 4 corners
-    // on that face are corners of the next cell too: their sums stay in
-    // registers (moved to the next cell's corner slots) and only the 4
-    // corners left behind are handed to `dep`.  The open cell's pose term
-    // includes the sums carried in from the previous cell; carry_pose()
-    // removes their share when they are carried into a new cell.
+    // (Density only.) The ray leaves the open cell through its face normal to
+    // axis k (s = +1 towards +k, -1 towards -k), into the neighbouring cell.
+    // The 4 corners on that face are corners of the next cell too: their sums
+    // stay in registers (moved to the next cell's corner slots) and only the 4
+    // corners left behind are handed to `dep`, lane by lane, so a face
+    // crossing costs half the LDS atomic lanes of a full flush.  (With the pose
+    // sums, a cell's pose term needs that cell's own sums, so the pose kernel
+    // flushes all 8 corners.)
     template <class Dep>
     __device__ __forceinline__ void leave_through_face(const VolDev &vol, Dep &dep, int k, int s) {
-        if (POSE) flush_pose();
         const int f = s < 0 ? 1 : 0;  // bit along k of the corners left behind
         // (masks and products instead of selects between array elements, which
         // the compiler would turn into indexed loads from scratch memory)
@@ -283,24 +286,6 @@ struct AdjRay {
         dep.part(vol, c.bx, c.by, c.bz, leave);
 #pragma unroll
         for (int d = 0; d < 8; d++) acc[d] = nacc[d];
-    }
-
-    // After the next cell is loaded: subtract the pose term of the sums just
-    // carried into it (flush_pose will count them with this cell's densities).
-    __device__ __forceinline__ void carry_pose() {
-        const float D[8] = {c.d0.x, c.d0.y, c.d0.z, c.d0.w, c.d1.x, c.d1.y, c.d1.z, c.d1.w};
-        float g0[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int b = 1 << k;
-#pragma unroll
-            for (int d = 0; d < 8; d++) {
-                if (d & b) continue;
-                g0[k] = fmaf(D[d | b] - D[d], acc[d] + acc[d | b], g0[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) add_S(k, -(double)g0[k]);
     }
 
     // March on while the floor of the sample position along `axis` of each
@@ -327,9 +312,10 @@ struct AdjRay {
         bool cont = cont_test();
         while (cont) {            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
                 // one step to a face neighbour (in clamped cell coordinates): carry
-                // the shared face; anything else: hand over all 8 corners
+                // the shared face (density-only kernel); anything else: hand over
+                // all 8 corners
                 int dd[3] = {0, 0, 0};
-                if (c.have) {
+                if (!POSE && c.have) {
                     dd[0] = clamp_cell(bx, vol.nx) - clamp_cell(c.bx, vol.nx);
                     dd[1] = clamp_cell(by, vol.ny) - clamp_cell(c.by, vol.ny);
                     dd[2] = clamp_cell(bz, vol.nz) - clamp_cell(c.bz, vol.nz);
@@ -342,7 +328,6 @@ struct AdjRay {
                     flush(vol, dep);
                 }
                 c.load(vol, px, py, pz, bx, by, bz);
-                if (POSE && face) carry_pose();
                 set_h();
             }
             const float fx = px - bx, fy = py - by, fz = pz - bz;
