@@ -197,8 +197,9 @@ def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1, repeats=3):
         return ms.value
 
     out = {}
-    for mode in ("pose", "density"):
-        theta = torch.zeros(6, device="cuda", requires_grad=(mode == "pose"))
+    for mode in ("pose", "density", "pose_only"):
+        theta = torch.zeros(6, device="cuda", requires_grad=(mode != "density"))
+        dens.requires_grad_(mode != "pose_only")
         fw, bw, fk, bk = [], [], [], []
         for it in range(repeats + 1):  # first one warms up
             torch.cuda.synchronize()
@@ -215,6 +216,7 @@ def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1, repeats=3):
             t3 = time.perf_counter()
             kb = kernel_ms()
             dens.grad = None
+            theta.grad = None
             if it:
                 fw.append(t1 - t0)
                 bw.append(t3 - t2)
@@ -225,11 +227,13 @@ def measure_adjoint(torch, n=192, n_views=64, det=256, step=0.1, repeats=3):
     dp.close()
     fwd, bwd, fk, bk = out["pose"]
     _, bwd_d, _, bk_d = out["density"]
+    _, bwd_p, _, bk_p = out["pose_only"]
     return dict(volume=f"{n}^3", views=n_views, detector=f"{det}^2", forward_ms_per_view=fwd,
                 adjoint_ms_per_view=bwd, adjoint_over_forward=bwd / fwd,
                 adjoint_density_only_ms_per_view=bwd_d, adjoint_density_only_over_forward=bwd_d / fwd,
                 forward_kernel_ms_per_view=fk, adjoint_kernel_ms_per_view=bk,
                 adjoint_density_only_kernel_ms_per_view=bk_d,
+                adjoint_pose_only_ms_per_view=bwd_p, adjoint_pose_only_kernel_ms_per_view=bk_p,
                 note="wall time (median of 3) of one drr_project_jobs launch (forward, saving area "
                      "densities) and one drr_adjoint_jobs launch (backward) over all 64 views; kernel_ms "
                      "from HIP events around the launches; MSE against a target at a perturbed pose")

@@ -177,6 +177,7 @@ def _make_function():
                 ctx.vol = None
             ctx.shape = density.shape
             ctx.need_pose = theta.requires_grad
+            ctx.need_density = density.requires_grad
             return I
 
         @staticmethod
@@ -184,7 +185,9 @@ def _make_function():
             dp = ctx.dp
             grad_out = grad_out.contiguous().float()
             ni, nj, nk = ctx.shape
-            gD = torch.zeros((nk, nj, ni), dtype=torch.float32, device=dp.device)
+            # pose-only backward (fixed density, e.g. registration) scatters nothing
+            gD = (torch.zeros((nk, nj, ni), dtype=torch.float32, device=dp.device)
+                  if ctx.need_density else None)
             gP = torch.zeros(6, dtype=torch.float32, device=dp.device) if ctx.need_pose else None
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
             c = np.asarray(dp.center, np.float32)
@@ -197,7 +200,7 @@ def _make_function():
                 ctx.vol = None
                 ctx.AD = None
             grad_theta = gP if ctx.need_pose else None
-            return gD.permute(2, 1, 0), grad_theta, None, None
+            return (gD.permute(2, 1, 0) if gD is not None else None), grad_theta, None, None
 
     return DRRFunction
 

@@ -77,6 +77,15 @@ struct GlobalDep {
     }
 };
 
+// Nowhere: the pose gradient alone (the density is not differentiated).
+struct NullDep {
+    __device__ __forceinline__ void operator()(const VolDev &, float, float, float, float (&acc)[8]) {
+#pragma unroll
+        for (int d = 0; d < 8; d++) acc[d] = 0.0f;
+    }
+    __device__ __forceinline__ void part(const VolDev &, float, float, float, const float (&)[8]) {}
+};
+
 // Into a workgroup's LDS box [o, o + L) (voxels, per axis); corners outside it
 // go to the gradient volume directly, so the box only changes the speed.
 struct BoxDep {
@@ -662,6 +671,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE
     });
 }
 
+// Pose gradient alone (registration with a fixed density): the same rays and
+// pose sums as k_adjoint_jobs<..., true>, without any density scatter, so no
+// slabs or LDS box.
+template <int M, bool ATTEN>
+__global__ __launch_bounds__(256) void k_adjoint_pose_jobs(const JobDev *__restrict__ jobs,
+                                                           const int2 *__restrict__ items,
+                                                           const int *__restrict__ seg, int *counters,
+                                                           const float *__restrict__ sw_jobs, VolDev vol,
+                                                           SpectrumDev sp, float step, int air, AdjArgs a,
+                                                           double *pose64) {
+    __shared__ double s_pose[4][6];
+    __shared__ double pose_slots[6 * 256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for_each_block_item(jobs, items, seg, counters, [&](int j, const JobDev &J, int tile) __attribute__((always_inline)) {
+        const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
+        int udx, vdx;
+        lane_pixel(tx * 2 + (wave & 1), ty * 2 + (wave >> 1), lane, udx, vdx);
+        double gpose[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        float gi = 0.0f, gp = 0.0f;
+        if (udx < J.g.width && vdx < J.g.height) {
+            const int64_t o = J.out_offset + (int64_t)vdx * J.g.width + udx;
+            gi = a.gI ? a.gI[o] : 0.0f;
+            gp = a.gP ? a.gP[o] : 0.0f;
+        }
+        if (gi != 0.0f || gp != 0.0f) {
+            float graw[M];
+            if (a.AD) {
+                const int64_t npix = (int64_t)J.g.width * J.g.height;
+                const float *ad_p = a.AD + M * J.out_offset + (int64_t)vdx * J.g.width + udx;
+                float ad[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) ad[m] = ad_p[m * npix];
+                graw_from_ad<M>(sp, ad, step, gi, gp, graw);
+            } else {
+                backward_spectral<M, ATTEN>(J.g, vol, sp, step, air, udx, vdx, gi, gp, graw);
+            }
+            AdjRay<M, true> ray;
+            ray.S = pose_slots + threadIdx.x;
+            ray.begin(J.g, vol, udx, vdx, step, graw);
+            NullDep dep;
+            ray.run(J.g, vol, step, 0, dep, AllCells{});
+            const float sw[3] = {sw_jobs[3 * j], sw_jobs[3 * j + 1], sw_jobs[3 * j + 2]};
+            ray.end_pose(J.g, sw, a.c, gpose);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            double v = gpose[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) s_pose[wave][q] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const double v = s_pose[0][threadIdx.x] + s_pose[1][threadIdx.x] + s_pose[2][threadIdx.x] +
+                             s_pose[3][threadIdx.x];
+            if (v != 0.0) unsafeAtomicAdd(pose64 + threadIdx.x, v);
+        }
+        __syncthreads();
+    });
+}
+
 template <int M, bool ATTEN>
 static void launch_adj(bool pose, dim3 gr, dim3 bl, hipStream_t st, const Geo1 &g, const VolDev &v,
                        const SpectrumDev &sp, float step, int air, const AdjArgs &a, const float *sw, int tx) {
@@ -675,7 +744,10 @@ template <int M, bool ATTEN>
 static void launch_adj_jobs(bool pose, const JobTable &t, hipStream_t st, const float *sw_jobs, const VolDev &v,
                             const SpectrumDev &sp, float step, int air, const AdjArgs &a, double *pose64) {
     dim3 gr(t.grid), bl(t.block);
-    if (pose)
+    if (!a.gD)
+        k_adjoint_pose_jobs<M, ATTEN><<<gr, bl, 0, st>>>(t.d_jobs, t.d_items, t.d_seg, t.d_counters, sw_jobs, v, sp,
+                                                         step, air, a, pose64);
+    else if (pose)
         k_adjoint_jobs<M, ATTEN, true><<<gr, bl, 0, st>>>(t.d_jobs, t.d_items, t.d_seg, t.d_counters, sw_jobs, v,
                                                           sp, step, air, a, pose64);
     else
@@ -684,11 +756,13 @@ static void launch_adj_jobs(bool pose, const JobTable &t, hipStream_t st, const 
 }
 
 static int check_adj(drr_ctx *ctx, const drr_volume *vol, const float *gI, const float *gP, const float *gD,
-                     const float *gPose, const float *sw, const float *c) {
+                     const float *gPose, const float *sw, const float *c, bool pose_only_ok = false) {
     if (!ctx || !vol) return drr_set_error(DRR_ERR_INVALID, "drr_adjoint: NULL context or volume");
     if (vol->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "drr_adjoint: volume of another context");
     if (!ctx->have_scene || ctx->n_bins < 1) return drr_set_error(DRR_ERR_STATE, "drr_adjoint: scene/spectrum not set");
-    if (!gD) return drr_set_error(DRR_ERR_INVALID, "drr_adjoint: grad_density is NULL");
+    if (!gD && !(pose_only_ok && gPose))
+        return drr_set_error(DRR_ERR_INVALID, pose_only_ok ? "drr_adjoint_jobs: neither grad_density nor grad_pose given"
+                                                           : "drr_adjoint: grad_density is NULL");
     if (!gI && !gP) return drr_set_error(DRR_ERR_INVALID, "drr_adjoint: no output gradient given");
     if (gPose && (!sw || !c))
         return drr_set_error(DRR_ERR_INVALID, "drr_adjoint: pose gradient needs source_world and pose_center_world");
@@ -753,7 +827,7 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
                                 const float *source_world, const float *pose_center_world,
                                 const float *area_density, float *grad_density, float *grad_pose, void *stream) {
     int rc = check_adj(ctx, vol, grad_intensity, grad_photon_prob, grad_density, grad_pose, source_world,
-                       pose_center_world);
+                       pose_center_world, true);
     if (rc) return rc;
     if (!jobs || n_jobs < 1) return drr_set_error(DRR_ERR_INVALID, "drr_adjoint_jobs: no jobs");
     for (int j = 0; j < n_jobs; j++)
@@ -767,7 +841,8 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
     // 16x16-pixel workgroup tiles, 4 waves per workgroup; as many persistent
     // workgroups per CU as the kernel's occupancy target holds (LDS allows it)
     const bool pose = grad_pose != nullptr;
-    const drr_batch_tuning_t tune{ADJ_TILE, ADJ_TILE, 4, pose ? AdjOcc<1>::waves : AdjOcc<0>::waves};
+    const int blocks_per_cu = pose ? AdjOcc<1>::waves : AdjOcc<0>::waves;  // pose only: 4, as with density
+    const drr_batch_tuning_t tune{ADJ_TILE, ADJ_TILE, 4, blocks_per_cu};
     rc = drr_jobs_build_tiles(ctx, vols, 1, jobs, n_jobs, tune, t);
     if (rc != DRR_OK) return rc;
     // scratch: FP64 pose accumulator [6], then the job sources [3 * n_jobs]

@@ -243,6 +243,9 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
         _abi.call("drr_adjoint_jobs", dp.ctx, h, jobs, len(views), _abi.ptr(gI), _abi.ptr(gPp),
                   src.ctypes.data, c.ctypes.data, _abi.ptr(AD_j), _abi.ptr(gD_r), _abi.ptr(gT_r),
                   _abi.stream_ptr())
+        gT_p = torch.zeros(6, device=gpu)
+        _abi.call("drr_adjoint_jobs", dp.ctx, h, jobs, len(views), _abi.ptr(gI), _abi.ptr(gPp),
+                  src.ctypes.data, c.ctypes.data, _abi.ptr(AD_j), None, _abi.ptr(gT_p), _abi.stream_ptr())
         gD_s = torch.zeros_like(d_kji)
         gT_s = torch.zeros(6, device=gpu)
         for i, (v, s) in enumerate(views):
@@ -263,6 +266,9 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
         r_ = gD_r.cpu().numpy()
         assert np.linalg.norm(r_ - a) <= 1e-6 * np.linalg.norm(a)
+        # pose alone (no density gradient): the same pose gradient
+        np.testing.assert_allclose(gT_p.cpu().numpy(), gT_r.cpu().numpy(), rtol=1e-5,
+                                   atol=1e-6 * float(np.abs(gT_r.cpu().numpy()).max()))
         np.testing.assert_allclose(gT_r.cpu().numpy(), gT_j.cpu().numpy(), rtol=1e-5,
                                    atol=1e-6 * float(np.abs(gT_j.cpu().numpy()).max()))
         np.testing.assert_allclose(gT_j.cpu().numpy(), gT_s.cpu().numpy(), rtol=1e-4,
