@@ -4,8 +4,9 @@
 
 Each ``csrc/*.hip`` is compiled to ``_build/*.o`` in parallel, then linked
 into ``deepdrr_amd/libdeepdrr_amd.so``.  Flags that matter for parity:
-``-ffp-contract=off`` (no FMA contraction: the kernels keep the reference's
-FP32 operation order), IEEE-correct FP32 division and square root, no
+``-ffp-contract=off`` (no implicit FMA contraction: the kernels keep the
+reference's FP32 operation order and fuse only the multiply-adds they write
+as ``__builtin_fmaf``, as the oracle does with ``fmaf``), IEEE-correct FP32 division and square root, no
 flush-to-zero, and no packed-FP32 instructions.
 """
 from __future__ import annotations

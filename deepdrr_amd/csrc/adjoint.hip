@@ -227,9 +227,9 @@ struct AdjRay {
         const float lim = r.mnv - (1.0f / rdom + 2.0f * step);
         skip_steps(r.N, lim, step, alpha, t);
         while (t < r.N && alpha < r.mnv) {
-            const float px = sx + alpha * rix - 1.0f;
-            const float py = sy + alpha * riy - 1.0f;
-            const float pz = sz + alpha * riz - 1.0f;
+            const float px = sample_pos(sx, alpha, rix);
+            const float py = sample_pos(sy, alpha, riy);
+            const float pz = sample_pos(sz, alpha, riz);
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
             if (!c.have || bx != c.bx || by != c.by || bz != c.bz) c.load(vol, px, py, pz, bx, by, bz);
             alpha += step;
@@ -309,7 +309,7 @@ struct AdjRay {
         // The sample position is computed one step ahead so that the slab test
         // sits at the top of the loop (one loop exit, before any state
         // changes, keeps the state in registers across the caller's slab loop).
-        float px = sx + alpha * rix - 1.0f, py = sy + alpha * riy - 1.0f, pz = sz + alpha * riz - 1.0f;
+        float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy), pz = sample_pos(sz, alpha, riz);
         float bx = floorf(px), by = floorf(py), bz = floorf(pz);
         // one branch-free continuation test, evaluated at the bottom of the loop
         auto cont_test = [&]() __attribute__((always_inline)) {
@@ -376,9 +376,9 @@ struct AdjRay {
             }
             alpha += step;
             ++t;
-            px = sx + alpha * rix - 1.0f;
-            py = sy + alpha * riy - 1.0f;
-            pz = sz + alpha * riz - 1.0f;
+            px = sample_pos(sx, alpha, rix);
+            py = sample_pos(sy, alpha, riy);
+            pz = sample_pos(sz, alpha, riz);
             bx = floorf(px);
             by = floorf(py);
             bz = floorf(pz);

@@ -3,8 +3,12 @@
 //
 // Semantics are those of deepdrr's projectKernel
 // (deepdrr/projector/project_kernel.cu:195-646): same FP32 operation order,
-// compiled with -ffp-contract=off, IEEE division and sqrt.  Differences are
-// in *how* the work is done, never in the values:
+// IEEE division and sqrt, and no implicit contraction (-ffp-contract=off);
+// three multiply-adds per sample are fused explicitly, where nvcc's default
+// --fmad=true fuses the reference's expressions: the sample position
+// (402-404), the area-density accumulation (543-547) and, in this engine's
+// exact-weight trilinear filter, its running sum.  Differences are in *how*
+// the work is done, never in the values:
 //   * the 8 label corners and 8 density corners of a trilinear cell are read
 //     once per cell change from the cell-expanded layout (drr_internal.h) and
 //     kept in registers while the ray stays in the cell; the reference keeps
@@ -16,8 +20,9 @@
 //     before entry, which reproduces the cache state of the reference (whose
 //     fetch point inside a cell decides the labels when p + 1 rounds up);
 //   * a cell whose 8 labels agree sums the weights once instead of per
-//     material; adding 0.0f to a non-negative accumulator is exact, so the
-//     result is bit-identical to the per-material sums (451-455, 543-547).
+//     material; a fused multiply-add with a zero fraction leaves an
+//     accumulator unchanged, so the result is bit-identical to the
+//     per-material sums (451-455, 543-547).
 #pragma once
 
 #include "drr_internal.h"
@@ -68,6 +73,13 @@ __device__ __forceinline__ bool slab(const float *A, const float *s, const float
         }
     }
     return true;
+}
+
+// Sample position along one axis: the reference's s + alpha * r - 1.0f, with
+// the multiply-add fused as nvcc fuses it by default (--fmad=true); the
+// oracle uses the same fmaf.
+__device__ __forceinline__ float sample_pos(float s, float alpha, float r) {
+    return __builtin_fmaf(alpha, r, s) - 1.0f;
 }
 
 __device__ __forceinline__ int clamp_cell(float b, int n) {
@@ -170,16 +182,17 @@ __device__ __forceinline__ void material_fractions(uint32_t l0, uint32_t l1, con
 }
 
 // Density at the sample: corner densities weighted by the same exact FP32
-// weights, summed in corner order starting from the first product.
+// weights, summed in corner order starting from the first product, each
+// further term fused into the running sum.
 __device__ __forceinline__ float trilerp(const float4 &d0, const float4 &d1, const float w[8]) {
     float rho = w[0] * d0.x;
-    rho += w[1] * d0.y;
-    rho += w[2] * d0.z;
-    rho += w[3] * d0.w;
-    rho += w[4] * d1.x;
-    rho += w[5] * d1.y;
-    rho += w[6] * d1.z;
-    rho += w[7] * d1.w;
+    rho = __builtin_fmaf(w[1], d0.y, rho);
+    rho = __builtin_fmaf(w[2], d0.z, rho);
+    rho = __builtin_fmaf(w[3], d0.w, rho);
+    rho = __builtin_fmaf(w[4], d1.x, rho);
+    rho = __builtin_fmaf(w[5], d1.y, rho);
+    rho = __builtin_fmaf(w[6], d1.z, rho);
+    rho = __builtin_fmaf(w[7], d1.w, rho);
     return rho;
 }
 
@@ -308,9 +321,9 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
         // Phase 2: approach window, label cache tracked as the reference does.
         Cell c;
         while (t < N && alpha < r.mnv) {
-            const float px = sx + alpha * rix - 1.0f;
-            const float py = sy + alpha * riy - 1.0f;
-            const float pz = sz + alpha * riz - 1.0f;
+            const float px = sample_pos(sx, alpha, rix);
+            const float py = sample_pos(sy, alpha, riy);
+            const float pz = sample_pos(sz, alpha, riz);
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
             if (!c.have || bx != c.bx || by != c.by || bz != c.bz) c.load(vol, px, py, pz, bx, by, bz);
             if (ATTEN) acc = add_air(acc);
@@ -329,9 +342,9 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
             cur = get_m<M>(ad, cm);
         }
         while (t < N && alpha <= r.mxv) {
-            const float px = sx + alpha * rix - 1.0f;
-            const float py = sy + alpha * riy - 1.0f;
-            const float pz = sz + alpha * riz - 1.0f;
+            const float px = sample_pos(sx, alpha, rix);
+            const float py = sample_pos(sy, alpha, riy);
+            const float pz = sample_pos(sz, alpha, riz);
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
             if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
                 c.load(vol, px, py, pz, bx, by, bz);
@@ -351,12 +364,12 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
             const float x = wt * rho;
             const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
             // uniform cell: seg[m0] = ssum, all other fractions 0
-            cur += c.uni ? x * ssum : 0.0f;
+            cur = c.uni ? __builtin_fmaf(x, ssum, cur) : cur;
             if (!c.uni) {
                 float seg[M];
                 mixed_fractions<M>(c.l0, c.l1, w, seg);
 #pragma unroll
-                for (int m = 0; m < M; m++) ad[m] += x * seg[m];
+                for (int m = 0; m < M; m++) ad[m] = __builtin_fmaf(x, seg[m], ad[m]);
             }
             alpha += step;
             ++t;
@@ -420,9 +433,9 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
             p[i][0] = p[i][1] = p[i][2] = 0.0f;
             if (!vd.en[i]) continue;
             const VolDev &vol = vd.vol[i];
-            p[i][0] = vd.s[i][0] + alpha * ri[i][0] - 1.0f;
-            p[i][1] = vd.s[i][1] + alpha * ri[i][1] - 1.0f;
-            p[i][2] = vd.s[i][2] + alpha * ri[i][2] - 1.0f;
+            p[i][0] = sample_pos(vd.s[i][0], alpha, ri[i][0]);
+            p[i][1] = sample_pos(vd.s[i][1], alpha, ri[i][1]);
+            p[i][2] = sample_pos(vd.s[i][2], alpha, ri[i][2]);
             const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
             if (!have || bx != pbx || by != pby || bz != pbz) {
                 const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
@@ -471,7 +484,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
                     weights8(p[i][0] - bx, p[i][1] - by, p[i][2] - bz, w);
                     const float rho = trilerp(d0, d1, w);
 #pragma unroll
-                    for (int m = 0; m < M; m++) ad[m] += (weight * rho) * seg[i][m];
+                    for (int m = 0; m < M; m++) ad[m] = __builtin_fmaf(weight * rho, seg[i][m], ad[m]);
                 }
             }
         }

@@ -7,7 +7,11 @@
  *
  * It follows deepdrr/projector/project_kernel.cu:135-650 statement by
  * statement in FP32 with the same operation order, built with
- * -ffp-contract=off (no FMA contraction) and without fast-math:
+ * -ffp-contract=off (no implicit FMA contraction) and without fast-math.
+ * Three multiply-adds are fused explicitly with fmaf, where nvcc's default
+ * --fmad=true fuses the reference's expressions: the sample position
+ * s + alpha * r - 1 (402-404), the accumulation ad[m] += (w * rho) * seg[m]
+ * (543-547) and the running sum of the exact-weight trilinear filter below.
  *   ray setup            project_kernel.cu:195-234
  *   slab clip            project_kernel.cu:242-323
  *   step count           project_kernel.cu:334
@@ -26,7 +30,7 @@
  *   density (linear, clamp, fetched at p + 0.5): corners clamp(floor(p)) and
  *           clamp(floor(p) + 1) weighted by the same exact FP32 weights as the
  *           labels, summed in (dz, dy, dx) order starting from the first
- *           product.  (NVIDIA's texture unit uses
+ *           product, each further term fused (fmaf).  (NVIDIA's texture unit uses
  *           8-bit fixed-point weights; exact weights are this engine's
  *           definition.)
  *   The label cache starts empty (the reference's first fetch is skipped if
@@ -159,9 +163,9 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
         for (int vi = 0; vi < V; vi++) {
             if (view->vol[vi].enabled == 0) continue;
             const float *s = view->vol[vi].source_ijk;
-            px[vi] = s[0] + alpha * rxi[vi] - 1.0f;
-            py[vi] = s[1] + alpha * ryi[vi] - 1.0f;
-            pz[vi] = s[2] + alpha * rzi[vi] - 1.0f;
+            px[vi] = fmaf(alpha, rxi[vi], s[0]) - 1.0f;
+            py[vi] = fmaf(alpha, ryi[vi], s[1]) - 1.0f;
+            pz[vi] = fmaf(alpha, rzi[vi], s[2]) - 1.0f;
             for (int m = 0; m < M; m++) seg[vi][m] = 0.0f;
             float bx = floorf(px[vi]), by = floorf(py[vi]), bz = floorf(pz[vi]);
             if (!have_prev || bx != prev[0] || by != prev[1] || bz != prev[2]) {
@@ -236,9 +240,9 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
                     float rho = w[0] * density_at(&vols[vi], ix, iy, iz);
                     for (int d = 1; d < 8; d++) {
                         int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
-                        rho += w[d] * density_at(&vols[vi], ix + dx, iy + dy, iz + dz);
+                        rho = fmaf(w[d], density_at(&vols[vi], ix + dx, iy + dy, iz + dz), rho);
                     }
-                    for (int m = 0; m < M; m++) ad[m] += (weight * rho) * seg[vi][m];
+                    for (int m = 0; m < M; m++) ad[m] = fmaf(weight * rho, seg[vi][m], ad[m]);
                 }
             }
         }
