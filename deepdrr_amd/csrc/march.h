@@ -82,6 +82,23 @@ __device__ __forceinline__ float sample_pos(float s, float alpha, float r) {
     return __builtin_fmaf(alpha, r, s) - 1.0f;
 }
 
+// Volume arrays live in device global memory.  Pointers that reach a kernel
+// through a table in memory (the batch's volume list) are generic to the
+// compiler, which then emits flat loads (slower, and they also wait on the
+// LDS counter); reading through an address-space-1 pointer gives global loads.
+typedef const __attribute__((address_space(1))) float gfloat;
+typedef const __attribute__((address_space(1))) uint32_t guint;
+
+__device__ __forceinline__ float4 gload(const float4 *p, uint32_t i) {
+    gfloat *q = (gfloat *)p + 4 * (size_t)i;
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ uint2 gload(const uint2 *p, uint32_t i) {
+    guint *q = (guint *)p + 2 * (size_t)i;
+    return make_uint2(q[0], q[1]);
+}
+
 __device__ __forceinline__ int clamp_cell(float b, int n) {
     // cell index in [-1, n-1]; clamp in float (one v_med3_f32) so that huge
     // values stay defined before the conversion
@@ -105,7 +122,7 @@ __device__ __forceinline__ uint32_t label_point(const VolDev &v, float x, float 
     const int i = clamp_voxel(floorf(x), v.nx);
     const int j = clamp_voxel(floorf(y), v.ny);
     const int k = clamp_voxel(floorf(z), v.nz);
-    return v.labels[cell_index(v, i, j, k)].x & 0xffu;
+    return gload(v.labels, cell_index(v, i, j, k)).x & 0xffu;
 }
 
 // Labels of the 8 corners for sample p with base b: L[clamp(floor(p + d))]
@@ -116,7 +133,7 @@ __device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py
     const bool regular = (floorf(px + 1.0f) == bx + 1.0f) && (floorf(py + 1.0f) == by + 1.0f) &&
                          (floorf(pz + 1.0f) == bz + 1.0f);
     if (regular) {
-        const uint2 lab = v.labels[cell];
+        const uint2 lab = gload(v.labels, cell);
         l0 = lab.x;
         l1 = lab.y;
     } else {
@@ -224,8 +241,8 @@ struct Cell {
                                          float y, float z) {
         const uint32_t cell = cell_index(vol, clamp_cell(x, vol.nx), clamp_cell(y, vol.ny), clamp_cell(z, vol.nz));
         fetch_labels(vol, px, py, pz, x, y, z, cell, l0, l1);
-        d0 = vol.dens[2 * cell];
-        d1 = vol.dens[2 * cell + 1];
+        d0 = gload(vol.dens, 2 * cell);
+        d1 = gload(vol.dens, 2 * cell + 1);
         m0 = l0 & 0xffu;
         uni = (l0 == l1) && (l0 == m0 * 0x01010101u);
         bx = x;
@@ -479,7 +496,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
                     const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
                     const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
                                                     clamp_cell(bz, vol.nz));
-                    const float4 d0 = vol.dens[2 * cell], d1 = vol.dens[2 * cell + 1];
+                    const float4 d0 = gload(vol.dens, 2 * cell), d1 = gload(vol.dens, 2 * cell + 1);
                     float w[8];
                     weights8(p[i][0] - bx, p[i][1] - by, p[i][2] - bz, w);
                     const float rho = trilerp(d0, d1, w);
