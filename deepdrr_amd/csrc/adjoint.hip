@@ -192,7 +192,7 @@ struct AdjRay {
     }
 
     __device__ __forceinline__ void set_h() {
-        if (c.uni) hu = get_m<M>(graw, (int)c.m0);
+        if (c.uni) hu = get_m<M>(graw, (int)c.l1);
     }
 
     // h_d of the open cell (a cell whose labels differ), recomputed where needed

@@ -5,7 +5,9 @@
 // (same for j, k), i.e. (nx+1)(ny+1)(nz+1) cells, i fastest:
 //   dens[cell]  : 8 float32 corner densities, corner d = dx + 2 dy + 4 dz
 //                 holding D[clamp(i+dx)][clamp(j+dy)][clamp(k+dz)]   (32 B)
-//   labels[cell]: 8 uint8 corner labels, same corner order           (8 B)
+//   labels[cell]: x = the 8 corner labels as 4-bit fields, corner d at
+//                 bits 4d..4d+3 (labels are < DRR_MAX_MATERIALS = 8);
+//                 y = their common label if all 8 agree, else ~0     (8 B)
 // A ray sample with base = floor(p) reads cell clamp(base, -1, n-1): two
 // aligned 16-byte loads and one 8-byte load replace the 16 scattered texel
 // fetches of the reference's texture path (project_kernel.cu:421-425, 542).

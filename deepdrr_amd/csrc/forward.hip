@@ -12,7 +12,8 @@ using namespace drr;
 // ---- volume cell expansion --------------------------------------------------
 
 // One thread per cell (i, j, k) in [-1, n-1]^3, i fastest: writes the 8 clamped
-// corner densities (2 x 16 B) and labels (8 B).  Consecutive threads write
+// corner densities (2 x 16 B) and the label word pair (8 B: corner labels as
+// nibbles, then their common label or ~0; labels are < 8).  Consecutive threads write
 // consecutive records, so the stores are fully coalesced.
 __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ density,
                                                 const uint8_t *__restrict__ labels, int nx, int ny,
@@ -38,8 +39,14 @@ __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ densit
         }
         dens[2 * c] = make_float4(d[0], d[1], d[2], d[3]);
         dens[2 * c + 1] = make_float4(d[4], d[5], d[6], d[7]);
-        lab[c] = make_uint2(l[0] | (l[1] << 8) | (l[2] << 16) | (l[3] << 24),
-                            l[4] | (l[5] << 8) | (l[6] << 16) | (l[7] << 24));
+        uint32_t packed = 0;
+        bool same = true;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            packed |= l[q] << (4 * q);
+            same = same && (l[q] == l[0]);
+        }
+        lab[c] = make_uint2(packed, same ? l[0] : 0xffffffffu);
     }
 }
 

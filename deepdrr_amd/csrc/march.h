@@ -122,16 +122,19 @@ __device__ __forceinline__ uint32_t label_point(const VolDev &v, float x, float 
     const int i = clamp_voxel(floorf(x), v.nx);
     const int j = clamp_voxel(floorf(y), v.ny);
     const int k = clamp_voxel(floorf(z), v.nz);
-    return gload(v.labels, cell_index(v, i, j, k)).x & 0xffu;
+    return gload(v.labels, cell_index(v, i, j, k)).x & 0xfu;
 }
 
 // Labels of the 8 corners for sample p with base b: L[clamp(floor(p + d))]
-// (point sampling at p + d, project_kernel.cu:421-425).  Packed 4 per word.
+// (point sampling at p + d, project_kernel.cu:421-425), in the cell record's
+// form: l0 = corner labels as nibbles, l1 = their common label or ~0.
 __device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py, float pz,
                                              float bx, float by, float bz, uint32_t cell,
                                              uint32_t &l0, uint32_t &l1) {
-    const bool regular = (floorf(px + 1.0f) == bx + 1.0f) && (floorf(py + 1.0f) == by + 1.0f) &&
-                         (floorf(pz + 1.0f) == bz + 1.0f);
+    // floor(p + 1) is b + 1 unless p + 1 rounds up to b + 2 (p < b + 1 bounds it
+    // above, p >= b below), so one comparison per axis decides it.
+    const bool regular = ((px + 1.0f) != (bx + 2.0f)) && ((py + 1.0f) != (by + 2.0f)) &&
+                         ((pz + 1.0f) != (bz + 2.0f));
     if (regular) {
         const uint2 lab = gload(v.labels, cell);
         l0 = lab.x;
@@ -143,8 +146,10 @@ __device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py
             const float dx = (float)(d & 1), dy = (float)((d >> 1) & 1), dz = (float)(d >> 2);
             b[d] = label_point(v, px + dx, py + dy, pz + dz);
         }
-        l0 = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
-        l1 = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
+        l0 = 0;
+#pragma unroll
+        for (int d = 0; d < 8; d++) l0 |= b[d] << (4 * d);
+        l1 = (l0 == b[0] * 0x11111111u) ? b[0] : 0xffffffffu;
     }
 }
 
@@ -160,8 +165,8 @@ __device__ __forceinline__ void weights8(float fx, float fy, float fz, float w[8
     w[7] = fx * fy * fz;
 }
 
-__device__ __forceinline__ uint32_t corner_label(uint32_t l0, uint32_t l1, int d) {
-    return ((d < 4 ? l0 : l1) >> (8 * (d & 3))) & 0xffu;
+__device__ __forceinline__ uint32_t corner_label(uint32_t l0, uint32_t, int d) {
+    return (l0 >> (4 * d)) & 0xfu;
 }
 
 // seg[m] = sum over corners (dz, dy, dx order) of w[d] where label == m, for a
@@ -181,9 +186,8 @@ __device__ __forceinline__ void mixed_fractions(uint32_t l0, uint32_t l1, const 
 template <int M>
 __device__ __forceinline__ void material_fractions(uint32_t l0, uint32_t l1, const float w[8],
                                                    float seg[M]) {
-    const uint32_t m0 = l0 & 0xffu;
-    const bool uniform = (l0 == l1) && (l0 == m0 * 0x01010101u);
-    if (uniform) {
+    const uint32_t m0 = l1;  // the common label, ~0 if the labels differ
+    if (m0 != 0xffffffffu) {
         const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
 #pragma unroll
         for (int m = 0; m < M; m++) seg[m] = ((uint32_t)m == m0) ? ssum : 0.0f;
@@ -228,29 +232,66 @@ __device__ __forceinline__ void set_m(float (&a)[M], int idx, float val) {
 }
 
 // Cached trilinear cell of the ray: base, 8 labels, 8 densities and whether
-// the labels agree.  `load` reads it for sample p with base b.
+// the labels agree (l1 = their common label).  `load` reads it for sample p with base b.
 struct Cell {
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
     bool have = false;
     bool uni = false;
-    uint32_t m0 = 0;
-    uint32_t l0 = 0, l1 = 0;
+    uint32_t l0 = 0, l1 = 0;  // corner labels (nibbles); common label or ~0
     float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), d1 = make_float4(0.f, 0.f, 0.f, 0.f);
 
     __device__ __forceinline__ void load(const VolDev &vol, float px, float py, float pz, float x,
                                          float y, float z) {
         const uint32_t cell = cell_index(vol, clamp_cell(x, vol.nx), clamp_cell(y, vol.ny), clamp_cell(z, vol.nz));
         fetch_labels(vol, px, py, pz, x, y, z, cell, l0, l1);
-        d0 = gload(vol.dens, 2 * cell);
-        d1 = gload(vol.dens, 2 * cell + 1);
-        m0 = l0 & 0xffu;
-        uni = (l0 == l1) && (l0 == m0 * 0x01010101u);
+        gfloat *dp = (gfloat *)vol.dens + 8 * (size_t)cell;  // one address, two 16-byte loads
+        d0 = make_float4(dp[0], dp[1], dp[2], dp[3]);
+        d1 = make_float4(dp[4], dp[5], dp[6], dp[7]);
+        uni = l1 != 0xffffffffu;
         bx = x;
         by = y;
         bz = z;
         have = true;
     }
 };
+
+// Running area density of the current uniform material: ad[cm] lives in
+// `cur` across consecutive uniform cells of one material and is written back
+// only when the material changes (cm < 0: none).
+struct Accum {
+    float cur = 0.0f;
+    int cm = -1;
+};
+
+// One in-volume sample of the single-volume march at position p
+// (project_kernel.cu:394-547) with weight wt (0.5 at the ends of the ray, else 1).
+template <int M>
+__device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad)[M], Accum &a, float px,
+                                          float py, float pz, float wt) {
+    const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+    if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
+        c.load(vol, px, py, pz, bx, by, bz);
+        const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
+        if (ncm != a.cm) {
+            if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
+            a.cm = ncm;
+            a.cur = get_m<M>(ad, ncm);
+        }
+    }
+    float w[8];
+    weights8(px - bx, py - by, pz - bz, w);
+    const float rho = trilerp(c.d0, c.d1, w);
+    const float x = wt * rho;
+    const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
+    // uniform cell: seg[m0] = ssum, all other fractions 0
+    a.cur = c.uni ? __builtin_fmaf(x, ssum, a.cur) : a.cur;
+    if (!c.uni) {
+        float seg[M];
+        mixed_fractions<M>(c.l0, c.l1, w, seg);
+#pragma unroll
+        for (int m = 0; m < M; m++) ad[m] = __builtin_fmaf(x, seg[m], ad[m]);
+    }
+}
 
 // Occupancy target of the march kernels: up to 5 materials the march fits in
 // 64 VGPRs (8 waves per SIMD) without spills; more materials get more registers.
@@ -351,47 +392,34 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
         // Phase 3: inside [minAlpha_vol, maxAlpha_vol]: sample and accumulate.
         // While the ray is in a cell whose 8 labels agree (material m0), only
         // ad[m0] changes; it is kept in `cur` and written back on leaving, so
-        // every FP32 addition is the one the reference performs.
-        float cur = 0.0f;
-        int cm = -1;
+        // every FP32 addition is the one the reference performs.  The end
+        // samples t == 0 and t == N-1 carry weight 0.5 (project_kernel.cu:
+        // 541), the others 1.
+        Accum acc3;
         if (c.have && c.uni) {
-            cm = (int)c.m0;
-            cur = get_m<M>(ad, cm);
+            acc3.cm = (int)c.l1;
+            acc3.cur = get_m<M>(ad, acc3.cm);
         }
-        while (t < N && alpha <= r.mxv) {
-            const float px = sample_pos(sx, alpha, rix);
-            const float py = sample_pos(sy, alpha, riy);
-            const float pz = sample_pos(sz, alpha, riz);
-            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
-                c.load(vol, px, py, pz, bx, by, bz);
-                // ad[cm] lives in `cur` across consecutive uniform cells of one
-                // material; it is written back only when the material changes.
-                const int ncm = c.uni ? (int)c.m0 : -1;
-                if (ncm != cm) {
-                    if (cm >= 0) set_m<M>(ad, cm, cur);
-                    cm = ncm;
-                    cur = get_m<M>(ad, cm);
-                }
+        // `tlim` bounds the weight-1 samples; the last one re-enters the loop
+        // once with tlim = N and weight 0.5 (one copy of the loop body).
+        float wt = (0 == t) ? 0.5f : 1.0f;
+        int tlim = N - 1;
+        for (;;) {
+            while (t < tlim && alpha <= r.mxv) {
+                sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
+                             sample_pos(sz, alpha, riz), wt);
+                wt = 1.0f;
+                alpha += step;
+                ++t;
             }
-            float w[8];
-            weights8(px - bx, py - by, pz - bz, w);
-            const float rho = trilerp(c.d0, c.d1, w);
-            const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
-            const float x = wt * rho;
-            const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
-            // uniform cell: seg[m0] = ssum, all other fractions 0
-            cur = c.uni ? __builtin_fmaf(x, ssum, cur) : cur;
-            if (!c.uni) {
-                float seg[M];
-                mixed_fractions<M>(c.l0, c.l1, w, seg);
-#pragma unroll
-                for (int m = 0; m < M; m++) ad[m] = __builtin_fmaf(x, seg[m], ad[m]);
-            }
-            alpha += step;
-            ++t;
+            // a fresh comparison after the loop (an empty asm hides alpha's
+            // in-loop history), so the loop does not carry a second one
+            asm volatile("" : "+v"(alpha));
+            if (tlim != N - 1 || t != N - 1 || !(alpha <= r.mxv)) break;
+            tlim = N;
+            wt = 0.5f;
         }
-        if (cm >= 0) set_m<M>(ad, cm, cur);
+        if (acc3.cm >= 0) set_m<M>(ad, acc3.cm, acc3.cur);
         if (ATTEN) acc = get_m<M>(ad, air);
     }
     // Phase 4: the rest of the steps sample nothing.

@@ -329,7 +329,7 @@ __global__ void k_unexpand(const float4 *dens, const uint2 *labels, int nx, int 
         const int z = (int)(i / ((int64_t)nx * ny));
         const int64_t cell = ((int64_t)(z + 1) * (ny + 1) + (y + 1)) * (int64_t)(nx + 1) + (x + 1);
         out_d[i] = dens[2 * cell].x;
-        out_l[i] = (uint8_t)(labels[cell].x & 0xffu);
+        out_l[i] = (uint8_t)(labels[cell].x & 0xfu);
     }
 }
 
