@@ -63,7 +63,56 @@ struct VolDev {
     const float4 *dens;
     const uint2 *labels;
     int nx, ny, nz;
+    // Cell index in FP32 arithmetic for clamped cell coordinates (cx, cy, cz)
+    // (march.h, Cell::load): with e1 = cz * a1 + (cy * b1 + c1) and
+    // e2 = cy * b2 + (cx + c2) (parameters from vol_index_params),
+    //   (uint32)e1 * s1 + bits(e2) = index + 0x4B000000,
+    // every intermediate an exact integer below 2^24.
+    float a1, b1, c1, b2, c2;
+    uint32_t s1;
+    float cmax[3];  // nx - 1, ny - 1, nz - 1: the largest cell coordinate
+    uint64_t dens_b, lab_b;  // record addresses of index -0x4B000000
 };
+
+// Parameters of the FP32 cell index.  Form A ((nx+1)(ny+1) <= 2^23):
+// e1 = cz + 1, e2 = (cy+1)(nx+1) + (cx+1) + 2^23, s1 = (nx+1)(ny+1).  Form B
+// ((nz+1)(ny+1) <= 2^23): e1 = (cz+1)(ny+1) + (cy+1), e2 = cx + 1 + 2^23,
+// s1 = nx+1.  Dimensions <= 16384 and fewer than 2^31 cells leave no volume
+// outside both forms ((nx+1)(ny+1) > 2^23 forces nz+1 < 256).  bits(e2) of
+// an integer e2 in [2^23, 2^24) is 0x4B000000 + (e2 - 2^23); the bias is
+// taken off the base addresses instead of each index.
+inline VolDev vol_index_params(const float4 *dens, const uint2 *labels, int nx, int ny, int nz) {
+    VolDev d{};
+    d.dens = dens;
+    d.labels = labels;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.cmax[0] = (float)(nx - 1);
+    d.cmax[1] = (float)(ny - 1);
+    d.cmax[2] = (float)(nz - 1);
+    const double nx1 = nx + 1.0, ny1 = ny + 1.0, nz1 = nz + 1.0;
+    const double two23 = 8388608.0;
+    if (nx1 * ny1 <= two23) {
+        d.a1 = 1.0f;
+        d.b1 = 0.0f;
+        d.c1 = 1.0f;
+        d.b2 = (float)nx1;
+        d.c2 = (float)(nx1 + 1.0 + two23);
+        d.s1 = (uint32_t)(nx1 * ny1);
+    } else {  // then nz + 1 < 256, so (nz+1)(ny+1) < 2^23
+        d.a1 = (float)ny1;
+        d.b1 = 1.0f;
+        d.c1 = (float)(ny1 + 1.0);
+        d.b2 = 0.0f;
+        d.c2 = (float)(1.0 + two23);
+        d.s1 = (uint32_t)nx1;
+        (void)nz1;
+    }
+    d.dens_b = (uint64_t)(uintptr_t)dens - (uint64_t)0x4B000000u * 32u;
+    d.lab_b = (uint64_t)(uintptr_t)labels - (uint64_t)0x4B000000u * 8u;
+    return d;
+}
 
 // Everything the per-ray march needs for one view of up to V volumes.
 struct ViewDev {
@@ -113,7 +162,7 @@ struct SpectrumDev {
 };
 
 inline VolDev vol_to_dev(const drr_volume *v) {
-    return VolDev{v->dens, v->labels, v->nx, v->ny, v->nz};
+    return vol_index_params(v->dens, v->labels, v->nx, v->ny, v->nz);
 }
 
 inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, ViewDev &d) {
@@ -132,13 +181,9 @@ inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, View
         d.prio[k] = vv.priority;
         d.en[k] = vv.enabled;
         if (k < v.n_volumes && vols && vols[k]) {
-            d.vol[k].dens = vols[k]->dens;
-            d.vol[k].labels = vols[k]->labels;
-            d.vol[k].nx = vols[k]->nx;
-            d.vol[k].ny = vols[k]->ny;
-            d.vol[k].nz = vols[k]->nz;
+            d.vol[k] = vol_to_dev(vols[k]);
         } else {
-            d.vol[k] = VolDev{nullptr, nullptr, 1, 1, 1};
+            d.vol[k] = vol_index_params(nullptr, nullptr, 1, 1, 1);
         }
     }
 }

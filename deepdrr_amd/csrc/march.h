@@ -112,45 +112,20 @@ __device__ __forceinline__ uint32_t cell_index(const VolDev &v, int cx, int cy, 
            (uint32_t)(cx + 1);
 }
 
-__device__ __forceinline__ int clamp_voxel(float f, int n) {
-    const float c = fminf(fmaxf(f, 0.0f), (float)(n - 1));
-    return (int)c;
-}
-
-// Label of voxel (clamp(floor(x)), ...) = corner 0 of that voxel's cell.
-__device__ __forceinline__ uint32_t label_point(const VolDev &v, float x, float y, float z) {
-    const int i = clamp_voxel(floorf(x), v.nx);
-    const int j = clamp_voxel(floorf(y), v.ny);
-    const int k = clamp_voxel(floorf(z), v.nz);
-    return gload(v.labels, cell_index(v, i, j, k)).x & 0xfu;
-}
-
-// Labels of the 8 corners for sample p with base b: L[clamp(floor(p + d))]
-// (point sampling at p + d, project_kernel.cu:421-425), in the cell record's
-// form: l0 = corner labels as nibbles, l1 = their common label or ~0.
-__device__ __forceinline__ void fetch_labels(const VolDev &v, float px, float py, float pz,
-                                             float bx, float by, float bz, uint32_t cell,
-                                             uint32_t &l0, uint32_t &l1) {
-    // floor(p + 1) is b + 1 unless p + 1 rounds up to b + 2 (p < b + 1 bounds it
-    // above, p >= b below), so one comparison per axis decides it.
-    const bool regular = ((px + 1.0f) != (bx + 2.0f)) && ((py + 1.0f) != (by + 2.0f)) &&
-                         ((pz + 1.0f) != (bz + 2.0f));
-    if (regular) {
-        const uint2 lab = gload(v.labels, cell);
-        l0 = lab.x;
-        l1 = lab.y;
-    } else {
-        uint32_t b[8];
-#pragma unroll
-        for (int d = 0; d < 8; d++) {
-            const float dx = (float)(d & 1), dy = (float)((d >> 1) & 1), dz = (float)(d >> 2);
-            b[d] = label_point(v, px + dx, py + dy, pz + dz);
-        }
-        l0 = 0;
-#pragma unroll
-        for (int d = 0; d < 8; d++) l0 |= b[d] << (4 * d);
-        l1 = (l0 == b[0] * 0x11111111u) ? b[0] : 0xffffffffu;
-    }
+// Labels of the 8 corners of the sample at p with base b = floor(p): the
+// reference point-samples L[clamp(floor(p + d))] (project_kernel.cu:421-425).
+// Every p the march forms is fl(q - 1) for an FP32 q (sample_pos), and for
+// such p with |p| < 2^22 the sum p + 1 is exact: for q >= 0.5, q - 1 is exact
+// (Sterbenz, and 1 is a multiple of ulp(q) below 2^24), so fl(p + 1) = q; for
+// p <= -0.5, |p| - 1 is a multiple of ulp(p).  Hence floor(p + d) = b + d.
+// For |p| >= 2^22 the sample is over 2^22 - 16384 voxels outside the volume,
+// where b + 1, b + 2 (and b) all clamp to the same edge voxel.  So the labels
+// are always L[clamp(b + d)], which is exactly the cell record of
+// clamp(b, -1, n-1): l0 = corner labels as nibbles, l1 = common label or ~0.
+__device__ __forceinline__ void fetch_labels(const VolDev &v, uint32_t cell, uint32_t &l0, uint32_t &l1) {
+    const uint2 lab = gload(v.labels, cell);
+    l0 = lab.x;
+    l1 = lab.y;
 }
 
 __device__ __forceinline__ void weights8(float fx, float fy, float fz, float w[8]) {
@@ -232,19 +207,28 @@ __device__ __forceinline__ void set_m(float (&a)[M], int idx, float val) {
 }
 
 // Cached trilinear cell of the ray: base, 8 labels, 8 densities and whether
-// the labels agree (l1 = their common label).  `load` reads it for sample p with base b.
+// the labels agree (l1 = their common label).  Before the first load the base
+// is far from any sample (the fractions p - b test as outside [0, 1)).
 struct Cell {
-    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    float bx = -1.0e30f, by = -1.0e30f, bz = -1.0e30f;
     bool have = false;
     bool uni = false;
     uint32_t l0 = 0, l1 = 0;  // corner labels (nibbles); common label or ~0
     float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), d1 = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    __device__ __forceinline__ void load(const VolDev &vol, float px, float py, float pz, float x,
-                                         float y, float z) {
-        const uint32_t cell = cell_index(vol, clamp_cell(x, vol.nx), clamp_cell(y, vol.ny), clamp_cell(z, vol.nz));
-        fetch_labels(vol, px, py, pz, x, y, z, cell, l0, l1);
-        gfloat *dp = (gfloat *)vol.dens + 8 * (size_t)cell;  // one address, two 16-byte loads
+    // Cell of base (x, y, z): clamped to [-1, n-1] per axis, its index formed
+    // in FP32 (VolDev: exact integers), records read from the biased bases.
+    __device__ __forceinline__ void load(const VolDev &vol, float x, float y, float z) {
+        const float cx = __builtin_amdgcn_fmed3f(x, -1.0f, vol.cmax[0]);
+        const float cy = __builtin_amdgcn_fmed3f(y, -1.0f, vol.cmax[1]);
+        const float cz = __builtin_amdgcn_fmed3f(z, -1.0f, vol.cmax[2]);
+        const float e1 = __builtin_fmaf(cz, vol.a1, __builtin_fmaf(cy, vol.b1, vol.c1));
+        const float e2 = __builtin_fmaf(cy, vol.b2, cx + vol.c2);
+        const uint32_t ib = (uint32_t)e1 * vol.s1 + __float_as_uint(e2);  // index + 0x4B000000
+        guint *lp = (guint *)(vol.lab_b + ((uint64_t)ib << 3));
+        gfloat *dp = (gfloat *)(vol.dens_b + ((uint64_t)ib << 5));
+        l0 = lp[0];
+        l1 = lp[1];
         d0 = make_float4(dp[0], dp[1], dp[2], dp[3]);
         d1 = make_float4(dp[4], dp[5], dp[6], dp[7]);
         uni = l1 != 0xffffffffu;
@@ -252,6 +236,9 @@ struct Cell {
         by = y;
         bz = z;
         have = true;
+    }
+    __device__ __forceinline__ void load(const VolDev &vol, float, float, float, float x, float y, float z) {
+        load(vol, x, y, z);
     }
 };
 
@@ -268,9 +255,17 @@ struct Accum {
 template <int M>
 __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad)[M], Accum &a, float px,
                                           float py, float pz, float wt) {
-    const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-    if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
-        c.load(vol, px, py, pz, bx, by, bz);
+    // floor(p) is the cached base exactly when every fraction p - b lies in
+    // [0, 1): p - b is exact there, and rounding is monotonic.  As unsigned
+    // bits, [0, 1) is [0, 0x3f800000); -0, negatives and NaN fall outside.
+    float fx = px - c.bx, fy = py - c.by, fz = pz - c.bz;
+    const uint32_t fmax = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz));
+    if (fmax >= 0x3f800000u) {
+        const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+        c.load(vol, bx, by, bz);
+        fx = px - bx;
+        fy = py - by;
+        fz = pz - bz;
         const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
         if (ncm != a.cm) {
             if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
@@ -279,7 +274,7 @@ __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad
         }
     }
     float w[8];
-    weights8(px - bx, py - by, pz - bz, w);
+    weights8(fx, fy, fz, w);
     const float rho = trilerp(c.d0, c.d1, w);
     const float x = wt * rho;
     const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
@@ -485,7 +480,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
             if (!have || bx != pbx || by != pby || bz != pbz) {
                 const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
                                                 clamp_cell(bz, vol.nz));
-                fetch_labels(vol, p[i][0], p[i][1], p[i][2], bx, by, bz, cell, l0, l1);
+                fetch_labels(vol, cell, l0, l1);
                 pbx = bx;
                 pby = by;
                 pbz = bz;
