@@ -310,16 +310,17 @@ struct AdjRay {
         // sits at the top of the loop (one loop exit, before any state
         // changes, keeps the state in registers across the caller's slab loop).
         float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy), pz = sample_pos(sz, alpha, riz);
-        float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+        // Fractions against the open cell: floor(p) is its base exactly when all
+        // three lie in [0, 1) (p - b is exact there; rounding is monotonic), i.e.
+        // their bits, unsigned, are below 0x3f800000 (a cell not yet loaded has
+        // a base of -1e30).  Only the slab axis needs the floor for the test.
+        float fx = px - c.bx, fy = py - c.by, fz = pz - c.bz;
+        bool same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
         // one branch-free continuation test, evaluated at the bottom of the loop
-        auto cont_test = [&]() __attribute__((always_inline)) {
-            const bool in_range = (t < N) & (alpha <= r.mxv);
-            const bool same = c.have & (bx == c.bx) & (by == c.by) & (bz == c.bz);
-            const bool ok = go(axis == 0 ? bx : (axis == 1 ? by : bz));
-            return in_range & (same | ok);
-        };
-        bool cont = cont_test();
-        while (cont) {            if (!c.have || bx != c.bx || by != c.by || bz != c.bz) {
+        bool cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
+        while (cont) {
+            if (!same) {
+                const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
                 // one step to a face neighbour (in clamped cell coordinates): carry
                 // the shared face (density-only kernel); anything else: hand over
                 // all 8 corners
@@ -336,13 +337,16 @@ struct AdjRay {
                 } else {
                     flush(vol, dep);
                 }
-                c.load(vol, px, py, pz, bx, by, bz);
+                c.load(vol, bx, by, bz);
                 set_h();
+                fx = px - bx;
+                fy = py - by;
+                fz = pz - bz;
             }
-            const float fx = px - bx, fy = py - by, fz = pz - bz;
             float w[8];
             weights8(fx, fy, fz, w);
-            const float wt = (0 == t || N - 1 == t) ? 0.5f : 1.0f;
+            // t == 0 or t == N-1 (t < N), as one unsigned comparison
+            const float wt = ((uint32_t)(t - 1) >= (uint32_t)(N - 2)) ? 0.5f : 1.0f;
             float sw;
             if (c.uni) {
                 const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
@@ -379,10 +383,11 @@ struct AdjRay {
             px = sample_pos(sx, alpha, rix);
             py = sample_pos(sy, alpha, riy);
             pz = sample_pos(sz, alpha, riz);
-            bx = floorf(px);
-            by = floorf(py);
-            bz = floorf(pz);
-            cont = cont_test();
+            fx = px - c.bx;
+            fy = py - c.by;
+            fz = pz - c.bz;
+            same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
+            cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
         }
         if (!(t < N && alpha <= r.mxv)) {
             flush(vol, dep);

@@ -172,3 +172,32 @@ def test_projector_multiview_and_errors(gpu):
     assert imgs.shape == (2, 6, 8)
     np.testing.assert_array_equal(imgs[0], imgs[1])
     p.free()
+
+
+def test_wide_slices(gpu):
+    """2900 x 2900 x 4 voxels: (nx+1)(ny+1) > 2^23, so the kernels form cell
+    indices the second way ((cz+1)(ny+1) + (cy+1) first); an oblique view
+    crosses cells along all three axes, and labels change in a checkerboard."""
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.volume import Volume
+
+    nx, ny, nz = 2900, 2900, 4
+    assert (nx + 1) * (ny + 1) > 1 << 23
+    rng = np.random.default_rng(11)
+    dens = (1.0 + 0.5 * rng.random((nx, ny, nz), dtype=np.float32)).astype(np.float32)
+    i = np.arange(nx)[:, None, None] // 37
+    j = np.arange(ny)[None, :, None] // 53
+    bone = ((i + j) % 2 == 0) & np.ones((1, 1, nz), bool)
+    spacing = (0.05, 0.05, 2.0)
+    vol = Volume.from_parameters(dens, {"bone": bone, "soft tissue": ~bone},
+                                 origin=phantoms._centered_origin((nx, ny, nz), spacing), spacing=spacing,
+                                 anatomical_coordinate_system="LPS")
+    d = np.array([0.2, 1.0, 0.05])
+    proj = phantoms.pose_projection(16, 12, 0.4, [0.0, 0.0, 0.0], d / np.linalg.norm(d), 0.5)
+    case = scenes.make_case("wide_slices", [vol], proj)
+    print("case wide_slices", flush=True)
+    r = GpuRunner(case)
+    try:
+        _check(case, r.project(), _oracle(case))
+    finally:
+        r.close()
