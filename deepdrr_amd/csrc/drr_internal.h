@@ -161,6 +161,17 @@ struct SpectrumDev {
     int n_bins;
 };
 
+// Index + 0x4B000000 of the cell at clamped coordinates (cx, cy, cz), each an
+// integer in [-1, n-1] held in FP32 (the march's Cell::load; host code runs it
+// too, to check it against integer arithmetic).
+__host__ __device__ __forceinline__ uint32_t cell_index_biased(const VolDev &v, float cx, float cy, float cz) {
+    const float e1 = __builtin_fmaf(cz, v.a1, __builtin_fmaf(cy, v.b1, v.c1));
+    const float e2 = __builtin_fmaf(cy, v.b2, cx + v.c2);
+    uint32_t bits;
+    __builtin_memcpy(&bits, &e2, 4);
+    return (uint32_t)e1 * v.s1 + bits;
+}
+
 inline VolDev vol_to_dev(const drr_volume *v) {
     return vol_index_params(v->dens, v->labels, v->nx, v->ny, v->nz);
 }

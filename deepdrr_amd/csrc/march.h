@@ -222,9 +222,7 @@ struct Cell {
         const float cx = __builtin_amdgcn_fmed3f(x, -1.0f, vol.cmax[0]);
         const float cy = __builtin_amdgcn_fmed3f(y, -1.0f, vol.cmax[1]);
         const float cz = __builtin_amdgcn_fmed3f(z, -1.0f, vol.cmax[2]);
-        const float e1 = __builtin_fmaf(cz, vol.a1, __builtin_fmaf(cy, vol.b1, vol.c1));
-        const float e2 = __builtin_fmaf(cy, vol.b2, cx + vol.c2);
-        const uint32_t ib = (uint32_t)e1 * vol.s1 + __float_as_uint(e2);  // index + 0x4B000000
+        const uint32_t ib = cell_index_biased(vol, cx, cy, cz);  // index + 0x4B000000
         guint *lp = (guint *)(vol.lab_b + ((uint64_t)ib << 3));
         gfloat *dp = (gfloat *)(vol.dens_b + ((uint64_t)ib << 5));
         l0 = lp[0];
