@@ -393,12 +393,20 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
             acc3.cm = (int)c.l1;
             acc3.cur = get_m<M>(ad, acc3.cm);
         }
-        // `tlim` bounds the weight-1 samples; the last one re-enters the loop
-        // once with tlim = N and weight 0.5 (one copy of the loop body).
+        // The loop runs while t < tlim with no test on alpha: tlim is at most
+        // the last weight-1 sample (N - 2) and at most the samples whose alpha
+        // is certainly <= maxAlpha_vol.  Each FP32 `alpha += step` adds at
+        // most step + 2^-24 (|mxv| + step) while alpha <= mxv, so with
+        // inc = step + 2^-23 (|mxv| + step) the next K = floor((mxv - alpha)
+        // / inc) - 2 samples stay inside (the -2 covers the FP32 quotient and
+        // difference).  Past tlim the loop re-enters one sample at a time with
+        // the full test; the last sample (t == N-1) re-enters with weight 0.5.
         float wt = (0 == t) ? 0.5f : 1.0f;
-        int tlim = N - 1;
+        const float inc = step + 1.1920929e-7f * (fabsf(r.mxv) + step);
+        const float kf = (r.mxv - alpha) / inc - 2.0f;
+        int tlim = t + (int)__builtin_amdgcn_fmed3f(kf, 0.0f, (float)(N - 1 - t));
         for (;;) {
-            while (t < tlim && alpha <= r.mxv) {
+            while (t < tlim) {
                 sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
                              sample_pos(sz, alpha, riz), wt);
                 wt = 1.0f;
@@ -406,11 +414,11 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
                 ++t;
             }
             // a fresh comparison after the loop (an empty asm hides alpha's
-            // in-loop history), so the loop does not carry a second one
+            // in-loop history), so the loop does not carry one
             asm volatile("" : "+v"(alpha));
-            if (tlim != N - 1 || t != N - 1 || !(alpha <= r.mxv)) break;
-            tlim = N;
-            wt = 0.5f;
+            if (!(alpha <= r.mxv) || t >= N) break;
+            if (t == N - 1) wt = 0.5f;
+            tlim = t + 1;
         }
         if (acc3.cm >= 0) set_m<M>(ad, acc3.cm, acc3.cur);
         if (ATTEN) acc = get_m<M>(ad, air);
