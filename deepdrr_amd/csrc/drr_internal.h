@@ -1,15 +1,15 @@
 // drr_internal.h -- shared definitions of the HIP library (not installed).
 //
 // Volume layout in HBM ("cell-expanded"): for a volume of nx*ny*nz voxels the
-// library stores one record per trilinear cell (i, j, k), i in [-1, nx-1]
-// (same for j, k), i.e. (nx+1)(ny+1)(nz+1) cells, i fastest:
-//   dens[cell]  : 8 float32 corner densities, corner d = dx + 2 dy + 4 dz
+// library stores one 40-byte record per trilinear cell (i, j, k), i in
+// [-1, nx-1] (same for j, k), i.e. (nx+1)(ny+1)(nz+1) cells, i fastest:
+//   words 0-7   : 8 float32 corner densities, corner d = dx + 2 dy + 4 dz
 //                 holding D[clamp(i+dx)][clamp(j+dy)][clamp(k+dz)]   (32 B)
-//   labels[cell]: x = the 8 corner labels as 4-bit fields, corner d at
-//                 bits 4d..4d+3 (labels are < DRR_MAX_MATERIALS = 8);
-//                 y = their common label if all 8 agree, else ~0     (8 B)
-// A ray sample with base = floor(p) reads cell clamp(base, -1, n-1): two
-// aligned 16-byte loads and one 8-byte load replace the 16 scattered texel
+//   word 8      : the 8 corner labels as 4-bit fields, corner d at bits
+//                 4d..4d+3 (labels are < DRR_MAX_MATERIALS = 8)
+//   word 9      : their common label if all 8 agree, else ~0          (8 B)
+// A ray sample with base = floor(p) reads cell clamp(base, -1, n-1): three
+// loads from one address (16 + 16 + 8 bytes) replace the 16 scattered texel
 // fetches of the reference's texture path (project_kernel.cu:421-425, 542).
 // Clamping the cell index reproduces clamp-to-edge addressing exactly.
 #pragma once
@@ -25,8 +25,7 @@ struct drr_volume_s {
     drr_ctx *ctx = nullptr;
     int nx = 0, ny = 0, nz = 0;
     int64_t n_cells = 0;
-    float4 *dens = nullptr;     // 2 float4 per cell
-    uint2 *labels = nullptr;    // 8 bytes per cell
+    uint32_t *cells = nullptr;  // DRR_CELL_WORDS 32-bit words per cell
     // voxel-order copies [nz][ny][nx], made on first use by the Monte-Carlo
     // path (nearest-voxel lookups read 5 bytes instead of a 40-byte cell)
     float *lin_dens = nullptr;
@@ -58,10 +57,11 @@ int drr_hip_error(hipError_t e, const char *what);
 
 // ---- kernel parameter blocks ------------------------------------------------
 
+#define DRR_CELL_WORDS 10  // words of one cell record: 8 densities, 2 label words
+
 // One volume as the march sees it.
 struct VolDev {
-    const float4 *dens;
-    const uint2 *labels;
+    const uint32_t *cells;
     int nx, ny, nz;
     // Cell index in FP32 arithmetic for clamped cell coordinates (cx, cy, cz)
     // (march.h, Cell::load): with e1 = cz * a1 + (cy * b1 + c1) and
@@ -71,7 +71,7 @@ struct VolDev {
     float a1, b1, c1, b2, c2;
     uint32_t s1;
     float cmax[3];  // nx - 1, ny - 1, nz - 1: the largest cell coordinate
-    uint64_t dens_b, lab_b;  // record addresses of index -0x4B000000
+    uint64_t cells_b;  // address of the record of index -0x4B000000
 };
 
 // Parameters of the FP32 cell index.  Form A ((nx+1)(ny+1) <= 2^23):
@@ -81,10 +81,9 @@ struct VolDev {
 // outside both forms ((nx+1)(ny+1) > 2^23 forces nz+1 < 256).  bits(e2) of
 // an integer e2 in [2^23, 2^24) is 0x4B000000 + (e2 - 2^23); the bias is
 // taken off the base addresses instead of each index.
-inline VolDev vol_index_params(const float4 *dens, const uint2 *labels, int nx, int ny, int nz) {
+inline VolDev vol_index_params(const uint32_t *cells, int nx, int ny, int nz) {
     VolDev d{};
-    d.dens = dens;
-    d.labels = labels;
+    d.cells = cells;
     d.nx = nx;
     d.ny = ny;
     d.nz = nz;
@@ -109,8 +108,7 @@ inline VolDev vol_index_params(const float4 *dens, const uint2 *labels, int nx, 
         d.s1 = (uint32_t)nx1;
         (void)nz1;
     }
-    d.dens_b = (uint64_t)(uintptr_t)dens - (uint64_t)0x4B000000u * 32u;
-    d.lab_b = (uint64_t)(uintptr_t)labels - (uint64_t)0x4B000000u * 8u;
+    d.cells_b = (uint64_t)(uintptr_t)cells - (uint64_t)0x4B000000u * (4u * DRR_CELL_WORDS);
     return d;
 }
 
@@ -173,7 +171,7 @@ __host__ __device__ __forceinline__ uint32_t cell_index_biased(const VolDev &v, 
 }
 
 inline VolDev vol_to_dev(const drr_volume *v) {
-    return vol_index_params(v->dens, v->labels, v->nx, v->ny, v->nz);
+    return vol_index_params(v->cells, v->nx, v->ny, v->nz);
 }
 
 inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, ViewDev &d) {
@@ -194,7 +192,7 @@ inline void view_to_dev(const drr_view_t &v, const drr_volume *const *vols, View
         if (k < v.n_volumes && vols && vols[k]) {
             d.vol[k] = vol_to_dev(vols[k]);
         } else {
-            d.vol[k] = vol_index_params(nullptr, nullptr, 1, 1, 1);
+            d.vol[k] = vol_index_params(nullptr, 1, 1, 1);
         }
     }
 }
@@ -209,4 +207,4 @@ int drr_volume_ensure_linear(drr_volume *vol);
 
 // Launchers implemented in forward.hip (used by batch/adjoint code).
 int drr_launch_expand(const float *density, const uint8_t *labels, int nx, int ny, int nz,
-                      float4 *dens, uint2 *lab, hipStream_t st);
+                      uint32_t *cells, hipStream_t st);

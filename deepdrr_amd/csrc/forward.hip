@@ -11,14 +11,14 @@ using namespace drr;
 
 // ---- volume cell expansion --------------------------------------------------
 
-// One thread per cell (i, j, k) in [-1, n-1]^3, i fastest: writes the 8 clamped
-// corner densities (2 x 16 B) and the label word pair (8 B: corner labels as
-// nibbles, then their common label or ~0; labels are < 8).  Consecutive threads write
-// consecutive records, so the stores are fully coalesced.
+// One thread per cell (i, j, k) in [-1, n-1]^3, i fastest: writes its 40-byte
+// record (drr_internal.h): the 8 clamped corner densities, the corner labels
+// as nibbles and their common label or ~0 (labels are < 8), as five 8-byte
+// stores.  Consecutive threads write consecutive records, so the stores are
+// fully coalesced.
 __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ density,
                                                 const uint8_t *__restrict__ labels, int nx, int ny,
-                                                int nz, float4 *__restrict__ dens,
-                                                uint2 *__restrict__ lab) {
+                                                int nz, uint32_t *__restrict__ cells) {
     const int64_t ncx = nx + 1, ncy = ny + 1, ncz = nz + 1;
     const int64_t ncell = ncx * ncy * ncz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell;
@@ -37,8 +37,6 @@ __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ densit
             d[q] = density[idx];
             l[q] = labels[idx];
         }
-        dens[2 * c] = make_float4(d[0], d[1], d[2], d[3]);
-        dens[2 * c + 1] = make_float4(d[4], d[5], d[6], d[7]);
         uint32_t packed = 0;
         bool same = true;
 #pragma unroll
@@ -46,16 +44,19 @@ __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ densit
             packed |= l[q] << (4 * q);
             same = same && (l[q] == l[0]);
         }
-        lab[c] = make_uint2(packed, same ? l[0] : 0xffffffffu);
+        uint2 *rec = reinterpret_cast<uint2 *>(cells + DRR_CELL_WORDS * c);  // 8-byte aligned
+#pragma unroll
+        for (int q = 0; q < 4; q++) rec[q] = make_uint2(__float_as_uint(d[2 * q]), __float_as_uint(d[2 * q + 1]));
+        rec[4] = make_uint2(packed, same ? l[0] : 0xffffffffu);
     }
 }
 
 int drr_launch_expand(const float *density, const uint8_t *labels, int nx, int ny, int nz,
-                      float4 *dens, uint2 *lab, hipStream_t st) {
+                      uint32_t *cells, hipStream_t st) {
     const int64_t ncell = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
     int64_t blocks = (ncell + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    k_expand<<<(unsigned)blocks, 256, 0, st>>>(density, labels, nx, ny, nz, dens, lab);
+    k_expand<<<(unsigned)blocks, 256, 0, st>>>(density, labels, nx, ny, nz, cells);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return drr_hip_error(e, "k_expand launch");
     return DRR_OK;

@@ -89,14 +89,9 @@ __device__ __forceinline__ float sample_pos(float s, float alpha, float r) {
 typedef const __attribute__((address_space(1))) float gfloat;
 typedef const __attribute__((address_space(1))) uint32_t guint;
 
-__device__ __forceinline__ float4 gload(const float4 *p, uint32_t i) {
-    gfloat *q = (gfloat *)p + 4 * (size_t)i;
-    return make_float4(q[0], q[1], q[2], q[3]);
-}
-
-__device__ __forceinline__ uint2 gload(const uint2 *p, uint32_t i) {
-    guint *q = (guint *)p + 2 * (size_t)i;
-    return make_uint2(q[0], q[1]);
+// Record of cell `cell` (DRR_CELL_WORDS words, drr_internal.h).
+__device__ __forceinline__ guint *cell_record(const VolDev &v, uint32_t cell) {
+    return (guint *)v.cells + DRR_CELL_WORDS * (size_t)cell;
 }
 
 __device__ __forceinline__ int clamp_cell(float b, int n) {
@@ -123,9 +118,9 @@ __device__ __forceinline__ uint32_t cell_index(const VolDev &v, int cx, int cy, 
 // are always L[clamp(b + d)], which is exactly the cell record of
 // clamp(b, -1, n-1): l0 = corner labels as nibbles, l1 = common label or ~0.
 __device__ __forceinline__ void fetch_labels(const VolDev &v, uint32_t cell, uint32_t &l0, uint32_t &l1) {
-    const uint2 lab = gload(v.labels, cell);
-    l0 = lab.x;
-    l1 = lab.y;
+    guint *r = cell_record(v, cell);
+    l0 = r[8];
+    l1 = r[9];
 }
 
 __device__ __forceinline__ void weights8(float fx, float fy, float fz, float w[8]) {
@@ -223,12 +218,11 @@ struct Cell {
         const float cy = __builtin_amdgcn_fmed3f(y, -1.0f, vol.cmax[1]);
         const float cz = __builtin_amdgcn_fmed3f(z, -1.0f, vol.cmax[2]);
         const uint32_t ib = cell_index_biased(vol, cx, cy, cz);  // index + 0x4B000000
-        guint *lp = (guint *)(vol.lab_b + ((uint64_t)ib << 3));
-        gfloat *dp = (gfloat *)(vol.dens_b + ((uint64_t)ib << 5));
-        l0 = lp[0];
-        l1 = lp[1];
-        d0 = make_float4(dp[0], dp[1], dp[2], dp[3]);
-        d1 = make_float4(dp[4], dp[5], dp[6], dp[7]);
+        gfloat *rp = (gfloat *)(vol.cells_b + (uint64_t)ib * (4u * DRR_CELL_WORDS));
+        d0 = make_float4(rp[0], rp[1], rp[2], rp[3]);
+        d1 = make_float4(rp[4], rp[5], rp[6], rp[7]);
+        l0 = ((guint *)rp)[8];
+        l1 = ((guint *)rp)[9];
         uni = l1 != 0xffffffffu;
         bx = x;
         by = y;
@@ -525,7 +519,9 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
                     const float bx = floorf(p[i][0]), by = floorf(p[i][1]), bz = floorf(p[i][2]);
                     const uint32_t cell = cell_index(vol, clamp_cell(bx, vol.nx), clamp_cell(by, vol.ny),
                                                     clamp_cell(bz, vol.nz));
-                    const float4 d0 = gload(vol.dens, 2 * cell), d1 = gload(vol.dens, 2 * cell + 1);
+                    gfloat *rp = (gfloat *)cell_record(vol, cell);
+                    const float4 d0 = make_float4(rp[0], rp[1], rp[2], rp[3]);
+                    const float4 d1 = make_float4(rp[4], rp[5], rp[6], rp[7]);
                     float w[8];
                     weights8(p[i][0] - bx, p[i][1] - by, p[i][2] - bz, w);
                     const float rho = trilerp(d0, d1, w);

@@ -176,11 +176,9 @@ static int volume_alloc(drr_ctx *ctx, int nx, int ny, int nz, drr_volume **out) 
     v->ny = ny;
     v->nz = nz;
     v->n_cells = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
-    hipError_t e = hipMalloc(&v->dens, sizeof(float4) * 2 * v->n_cells);
-    if (e == hipSuccess) e = hipMalloc(&v->labels, sizeof(uint2) * v->n_cells);
+    hipError_t e = hipMalloc(&v->cells, sizeof(uint32_t) * DRR_CELL_WORDS * v->n_cells);
     if (e != hipSuccess) {
-        (void)hipFree(v->dens);
-        (void)hipFree(v->labels);
+        (void)hipFree(v->cells);
         delete v;
         return drr_set_error(DRR_ERR_ALLOC, std::string("drr_volume: device allocation failed: ") +
                                                 hipGetErrorString(e));
@@ -239,7 +237,7 @@ int drr_volume_create(drr_ctx *ctx, const float *density, const uint8_t *labels,
     }
     drr_volume *v = nullptr;
     rc = volume_alloc(ctx, nx, ny, nz, &v);
-    if (rc == DRR_OK) rc = drr_launch_expand(d_den, d_lab, nx, ny, nz, v->dens, v->labels, 0);
+    if (rc == DRR_OK) rc = drr_launch_expand(d_den, d_lab, nx, ny, nz, v->cells, 0);
     if (rc == DRR_OK) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = drr_hip_error(e, "drr_volume_create: expand");
@@ -290,7 +288,7 @@ int drr_volume_create_from_hu(drr_ctx *ctx, const int16_t *hu, int nx, int ny, i
     DRR_HIP_CHECK(hipGetLastError());
     drr_volume *v = nullptr;
     rc = volume_alloc(ctx, nx, ny, nz, &v);
-    if (rc == DRR_OK) rc = drr_launch_expand(d_den, d_lab, nx, ny, nz, v->dens, v->labels, 0);
+    if (rc == DRR_OK) rc = drr_launch_expand(d_den, d_lab, nx, ny, nz, v->cells, 0);
     if (rc == DRR_OK) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = drr_hip_error(e, "drr_volume_create_from_hu");
@@ -311,16 +309,14 @@ int drr_volume_create_from_hu(drr_ctx *ctx, const int16_t *hu, int nx, int ny, i
 int drr_volume_destroy(drr_volume *vol) {
     if (!vol) return DRR_OK;
     DeviceGuard g(vol->ctx ? vol->ctx->device : 0);
-    (void)hipFree(vol->dens);
-    (void)hipFree(vol->labels);
+    (void)hipFree(vol->cells);
     (void)hipFree(vol->lin_dens);
     (void)hipFree(vol->lin_lab);
     delete vol;
     return DRR_OK;
 }
 
-__global__ void k_unexpand(const float4 *dens, const uint2 *labels, int nx, int ny, int nz,
-                           float *out_d, uint8_t *out_l) {
+__global__ void k_unexpand(const uint32_t *cells, int nx, int ny, int nz, float *out_d, uint8_t *out_l) {
     const int64_t n = (int64_t)nx * ny * nz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -328,8 +324,9 @@ __global__ void k_unexpand(const float4 *dens, const uint2 *labels, int nx, int 
         const int y = (int)((i / nx) % ny);
         const int z = (int)(i / ((int64_t)nx * ny));
         const int64_t cell = ((int64_t)(z + 1) * (ny + 1) + (y + 1)) * (int64_t)(nx + 1) + (x + 1);
-        out_d[i] = dens[2 * cell].x;
-        out_l[i] = (uint8_t)(labels[cell].x & 0xfu);
+        const uint32_t *rec = cells + DRR_CELL_WORDS * cell;  // corner 0 of the voxel's own cell
+        out_d[i] = __uint_as_float(rec[0]);
+        out_l[i] = (uint8_t)(rec[8] & 0xfu);
     }
 }
 
@@ -348,8 +345,7 @@ int drr_volume_ensure_linear(drr_volume *vol) {
     const int64_t n = (int64_t)vol->nx * vol->ny * vol->nz;
     DRR_HIP_CHECK(hipMalloc(&vol->lin_dens, sizeof(float) * n));
     DRR_HIP_CHECK(hipMalloc(&vol->lin_lab, n));
-    k_unexpand<<<2048, 256, 0, 0>>>(vol->dens, vol->labels, vol->nx, vol->ny, vol->nz, vol->lin_dens,
-                                    vol->lin_lab);
+    k_unexpand<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, vol->lin_dens, vol->lin_lab);
     DRR_HIP_CHECK(hipGetLastError());
     unsigned int *d_max = nullptr;
     DRR_HIP_CHECK(hipMalloc(&d_max, sizeof(unsigned int)));
@@ -375,7 +371,7 @@ int drr_volume_download(drr_volume *vol, float *density, uint8_t *labels) {
     uint8_t *l = nullptr;
     DRR_HIP_CHECK(hipMalloc(&d, sizeof(float) * n));
     DRR_HIP_CHECK(hipMalloc(&l, n));
-    k_unexpand<<<2048, 256, 0, 0>>>(vol->dens, vol->labels, vol->nx, vol->ny, vol->nz, d, l);
+    k_unexpand<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, d, l);
     DRR_HIP_CHECK(hipGetLastError());
     DRR_HIP_CHECK(hipMemcpy(density, d, sizeof(float) * n, hipMemcpyDeviceToHost));
     DRR_HIP_CHECK(hipMemcpy(labels, l, n, hipMemcpyDeviceToHost));

@@ -18,7 +18,7 @@ static uint32_t next_u32() {
 }
 
 static int check(int nx, int ny, int nz, long long *n_checked) {
-    const VolDev v = vol_index_params(nullptr, nullptr, nx, ny, nz);
+    const VolDev v = vol_index_params(nullptr, nx, ny, nz);
     const int n[3] = {nx, ny, nz};
     for (int trial = 0; trial < 4000; trial++) {
         int c[3];
