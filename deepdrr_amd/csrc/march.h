@@ -273,10 +273,20 @@ __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad
     // uniform cell: seg[m0] = ssum, all other fractions 0
     a.cur = c.uni ? __builtin_fmaf(x, ssum, a.cur) : a.cur;
     if (!c.uni) {
-        float seg[M];
-        mixed_fractions<M>(c.l0, c.l1, w, seg);
+        // Labels differ: per material present among the 8 corners (a zero
+        // nibble of l0 ^ m * 0x11111111, by the borrow test), its fraction in
+        // corner order; an absent material's fraction is 0 and leaves ad[m]
+        // unchanged (fma(x, 0, a) = a for a >= 0), so it is skipped.
 #pragma unroll
-        for (int m = 0; m < M; m++) ad[m] = __builtin_fmaf(x, seg[m], ad[m]);
+        for (int m = 0; m < M; m++) {
+            const uint32_t xm = c.l0 ^ (0x11111111u * (uint32_t)m);
+            if (((xm - 0x11111111u) & ~xm & 0x88888888u) != 0u) {
+                float sm = 0.0f;
+#pragma unroll
+                for (int d = 0; d < 8; d++) sm += (corner_label(c.l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+                ad[m] = __builtin_fmaf(x, sm, ad[m]);
+            }
+        }
     }
 }
 
