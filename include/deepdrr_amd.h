@@ -94,8 +94,9 @@ int drr_ctx_set_spectrum(drr_ctx *ctx, int n_bins, const float *energies_keV,
 /* Upload one volume.  density: float32 [nz][ny][nx] (i fastest), labels:
  * uint8 [nz][ny][nx] holding global material indices < n_materials.
  * `on_device` = 1 if both pointers are HIP device pointers.  The volume is
- * stored cell-expanded in HBM: for every cell (i, j, k) in [-1, n-1]^3 the 8
- * clamped corner densities (32 B) and 8 corner labels (8 B) side by side.
+ * stored cell-expanded in HBM: for every cell (i, j, k) in [-1, n-1]^3 one
+ * 40-byte record of the 8 clamped corner densities (32 B), the 8 corner
+ * labels as 4-bit fields and their common label (8 B).
  * Replaces create_cuda_texture (projector.py:116-257, 1464-1539). */
 int drr_volume_create(drr_ctx *ctx, const float *density, const uint8_t *labels,
                       int nx, int ny, int nz, int on_device, drr_volume **out);
