@@ -93,7 +93,13 @@ class BatchPlan:
         buf = {"images": self.images, "intensity": self.intensity, "photon_prob": self.photon_prob}[which]
         if buf is None:
             raise ValueError(f"{which} was not requested when the batch was planned")
-        host = buf.cpu().numpy()
+        # A page-locked destination lets the copy run as one DMA at link speed
+        # (a pageable one is staged through a bounce buffer).  Each call gets
+        # its own buffer, so earlier results stay valid; torch's pinned-memory
+        # cache hands back the block once those arrays are dropped.
+        host = self.torch.empty(buf.numel(), dtype=buf.dtype, pin_memory=True)
+        host.copy_(buf)
+        host = host.numpy()
         return [host[o:o + h * w].reshape(h, w) for o, (h, w) in zip(self.offsets, self.shapes)]
 
     def __call__(self, which: Optional[str] = None):

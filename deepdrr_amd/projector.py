@@ -34,6 +34,16 @@ class DeprecationError(Exception):
     pass
 
 
+def to_host(t) -> np.ndarray:
+    """Device tensor -> NumPy through a page-locked buffer (one DMA at link
+    speed; a pageable destination is staged through a bounce buffer)."""
+    import torch
+
+    host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    host.copy_(t)
+    return host.numpy()
+
+
 def volume_view_params(vol: Volume, proj: geo.CameraProjection, priority: int) -> dict:
     """Per-volume uniforms of one view (projector.py:802-831, 1626-1637)."""
     ijk_from_world = np.asarray(vol.IJK_from_world.toarray())
@@ -315,15 +325,15 @@ class Projector:
         _abi.call("drr_project", self._ctx, self._vol_array, C.byref(view),
                   _abi.ptr(self.intensity_gpu), _abi.ptr(self.photon_prob_gpu), None,
                   _abi.ptr(self.solid_angle_gpu), stream)
-        intensity = self.intensity_gpu.cpu().numpy()
-        photon_prob = self.photon_prob_gpu.cpu().numpy()
+        intensity = to_host(self.intensity_gpu)
+        photon_prob = to_host(self.photon_prob_gpu)
         if self.collected_energy:
             intensity = self._calculate_collected_energy_per_pixel(proj, intensity)
         return intensity, photon_prob
 
     def _calculate_collected_energy_per_pixel(self, proj, intensity):
         """projector.py:833-853."""
-        solid_angle = self.solid_angle_gpu.cpu().numpy()
+        solid_angle = to_host(self.solid_angle_gpu)
         px = self.source_to_detector_distance / proj.index_from_camera2d.fx
         py = self.source_to_detector_distance / proj.index_from_camera2d.fy
         deposited = np.multiply(intensity, solid_angle) * self.photon_count / np.average(solid_angle)
@@ -425,7 +435,7 @@ class Projector:
         if self.neglog:
             _abi.call("drr_neglog", self._ctx, _abi.ptr(noisy), _abi.ptr(noisy), n, h * w,
                       C.c_float(utils.NEGLOG_EPSILON), stream)
-        host = noisy.cpu().numpy()
+        host = to_host(noisy)
         return [host[i * h * w:(i + 1) * h * w].reshape(h, w) for i in range(n)]
 
     def project_area_density(self, proj: geo.CameraProjection):
@@ -444,4 +454,4 @@ class Projector:
         view = self.view_for(proj)
         _abi.call("drr_project", self._ctx, self._vol_array, C.byref(view), _abi.ptr(I), _abi.ptr(P),
                   _abi.ptr(AD), None, _abi.stream_ptr(torch.cuda.current_stream(self._torch_device)))
-        return AD.cpu().numpy(), I.cpu().numpy(), P.cpu().numpy()
+        return to_host(AD), to_host(I), to_host(P)
