@@ -408,7 +408,9 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
         float wt = (0 == t) ? 0.5f : 1.0f;
         const float inc = step + 1.1920929e-7f * (fabsf(r.mxv) + step);
         const float kf = (r.mxv - alpha) / inc - 2.0f;
-        int tlim = t + (int)__builtin_amdgcn_fmed3f(kf, 0.0f, (float)(N - 1 - t));
+        // (clamped in float to a range the conversion holds, then to N - 1 in
+        // integers, where (float)(N - 1 - t) could round up)
+        int tlim = min(t + (int)__builtin_amdgcn_fmed3f(kf, 0.0f, 16777216.0f), N - 1);
         for (;;) {
             while (t < tlim) {
                 sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
