@@ -201,3 +201,24 @@ def test_wide_slices(gpu):
         _check(case, r.project(), _oracle(case))
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("sdd", [2.0e4, 4.0e6])
+def test_far_source(gpu, sdd):
+    """A source far from the volume: alpha is large, so each FP32 `alpha += step`
+    rounds noticeably (at 2e6 mm by up to a quarter step).  The kernels bound the
+    samples they march without the maxAlpha test by that rounding and march the
+    rest one at a time with the full test; the result must still equal the
+    transcription's sequential sum bit for bit."""
+    from deepdrr_amd import phantoms
+
+    vol = phantoms.small_ct(32, seed=3)
+    dev = phantoms.ap_device(16, 12, 4.0, sdd=sdd)
+    case = scenes.make_case(f"far_source_{sdd:g}", [vol], dev.get_camera_projection(),
+                            max_ray_length=4.0 * sdd)
+    print(f"case {case.name}", flush=True)
+    r = GpuRunner(case)
+    try:
+        _check(case, r.project(), _oracle(case))
+    finally:
+        r.close()
