@@ -17,8 +17,8 @@
 //   * for a single volume the steps before the volume are skipped with the
 //     same sequential FP32 `alpha += step` (552) but without sampling; the
 //     label cache is re-established inside a window of at least one voxel
-//     before entry, which reproduces the cache state of the reference (whose
-//     fetch point inside a cell decides the labels when p + 1 rounds up);
+//     before entry, which reproduces the cache state of the reference (the
+//     labels a fetch returns depend on the cell alone, see fetch_labels);
 //   * a cell whose 8 labels agree sums the weights once instead of per
 //     material; a fused multiply-add with a zero fraction leaves an
 //     accumulator unchanged, so the result is bit-identical to the
