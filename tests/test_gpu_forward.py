@@ -222,3 +222,41 @@ def test_far_source(gpu, sdd):
         _check(case, r.project(), _oracle(case))
     finally:
         r.close()
+
+
+def _random_scene(seed):
+    """A small anisotropic volume with three labelled materials (boundaries
+    everywhere), a random pose and detector, a random step, with or without
+    attenuation outside the volume."""
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.volume import Volume
+
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(x) for x in rng.integers(3, 28, 3))
+    spacing = tuple(float(x) for x in rng.uniform(0.4, 2.5, 3))
+    dens = rng.uniform(0.0, 2.0, shape).astype(np.float32)
+    u = rng.random(shape)
+    bone = u < 0.3
+    air = (u >= 0.3) & (u < 0.5)
+    vol = Volume.from_parameters(dens, {"bone": bone, "air": air, "soft tissue": ~(bone | air)},
+                                 origin=phantoms._centered_origin(shape, spacing), spacing=spacing,
+                                 anatomical_coordinate_system="LPS")
+    (p, d, f), = phantoms.batch_poses(1, seed=seed, radius=0.3 * max(shape) * max(spacing), cone_deg=80.0)
+    w, h = (int(x) for x in rng.integers(4, 20, 2))
+    proj = phantoms.pose_projection(w, h, float(rng.uniform(0.5, 4.0)), p, d, f)
+    step = float(rng.choice([0.05, 0.1, 0.37, 1.3]))
+    return scenes.make_case(f"random{seed}", [vol], proj, step=step, atten=bool(rng.random() < 0.4))
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_scenes(gpu, seed):
+    """Bit-exact area densities on random anisotropic volumes, poses, detectors
+    and steps (the cell loads, the label identity, the FP32 cell index, the
+    loop bound and the tail, with and without air outside the volume)."""
+    case = _random_scene(seed)
+    print(f"case {case.name}", flush=True)
+    r = GpuRunner(case)
+    try:
+        _check(case, r.project(), _oracle(case))
+    finally:
+        r.close()
