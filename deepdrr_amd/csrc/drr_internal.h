@@ -26,10 +26,10 @@ struct drr_volume_s {
     int nx = 0, ny = 0, nz = 0;
     int64_t n_cells = 0;
     uint32_t *cells = nullptr;  // DRR_CELL_WORDS 32-bit words per cell
-    // voxel-order copies [nz][ny][nx], made on first use by the Monte-Carlo
-    // path (nearest-voxel lookups read 5 bytes instead of a 40-byte cell)
-    float *lin_dens = nullptr;
-    uint8_t *lin_lab = nullptr;
+    // voxel-order records [nz][ny][nx] of (density bits, label), made on first
+    // use by the Monte-Carlo path: a nearest-voxel lookup is one 8-byte load
+    // (one memory sector, where separate density and label arrays take two)
+    uint2 *lin_vox = nullptr;
     float max_density = 0.0f;
 };
 
@@ -202,7 +202,7 @@ int drr_check_view(const drr_ctx *ctx, const drr_volume *const *vols, const drr_
 // Every float of the view's geometry is finite (else DRR_ERR_INVALID).
 int drr_check_view_finite(const drr_view_t *view);
 
-// Make vol->lin_dens / lin_lab / max_density if absent (runtime.hip).
+// Make vol->lin_vox and vol->max_density if absent (runtime.hip).
 int drr_volume_ensure_linear(drr_volume *vol);
 
 // Launchers implemented in forward.hip (used by batch/adjoint code).

@@ -54,24 +54,26 @@ __device__ __forceinline__ float ranecu(Ranecu &r) {
     return fminf((float)z * 4.656612873077393e-10f, 0.99999994f);
 }
 
-// Rotate unit direction (u, v, w) by polar cos(theta) and azimuth phi.
+// Rotate unit direction (u, v, w) by polar cos(theta) and azimuth phi in
+// [0, 2 pi) (the hardware sine and cosine: a few ulp, ample for sampling).
 __device__ __forceinline__ void rotate(float &u, float &v, float &w, float ct, float phi) {
     const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-    const float cp = cosf(phi), sp = sinf(phi);
+    const float cp = __cosf(phi), sp = __sinf(phi);
     if (fabsf(w) > 0.99999f) {
         u = st * cp;
         v = st * sp;
         w = (w > 0.0f ? 1.0f : -1.0f) * ct;
     } else {
-        const float t = sqrtf(1.0f - w * w);
-        const float nu = u * ct + st * (u * w * cp - v * sp) / t;
-        const float nv = v * ct + st * (v * w * cp + u * sp) / t;
+        const float it = rsqrtf(1.0f - w * w), t = (1.0f - w * w) * it;
+        const float sit = st * it;
+        const float nu = u * ct + sit * (u * w * cp - v * sp);
+        const float nv = v * ct + sit * (v * w * cp + u * sp);
         const float nw = w * ct - t * st * cp;
         u = nu;
         v = nv;
         w = nw;
     }
-    const float n = 1.0f / sqrtf(u * u + v * v + w * w);
+    const float n = rsqrtf(u * u + v * v + w * w);
     u *= n;
     v *= n;
     w *= n;
@@ -79,7 +81,7 @@ __device__ __forceinline__ void rotate(float &u, float &v, float &w, float ct, f
 
 // Klein-Nishina by Kahn's method; returns cos(theta), updates E.
 __device__ __forceinline__ float compton(Ranecu &rng, float &E) {
-    const float k = E / 510.99895f;
+    const float k = E * (1.0f / 510.99895f), ik = 1.0f / k;
     const float b1 = (1.0f + 2.0f * k) / (9.0f + 2.0f * k);
     float eta, ct;
     for (int it = 0; it < 1000; it++) {
@@ -89,11 +91,11 @@ __device__ __forceinline__ float compton(Ranecu &rng, float &E) {
             if (r3 <= 4.0f * (1.0f / eta - 1.0f / (eta * eta))) break;
         } else {
             eta = (1.0f + 2.0f * k) / (1.0f + 2.0f * k * r2);
-            ct = 1.0f - (eta - 1.0f) / k;
+            ct = 1.0f - (eta - 1.0f) * ik;
             if (r3 <= 0.5f * (ct * ct + 1.0f / eta)) break;
         }
     }
-    ct = 1.0f - (eta - 1.0f) / k;
+    ct = 1.0f - (eta - 1.0f) * ik;
     E = E / eta;
     return fminf(1.0f, fmaxf(-1.0f, ct));
 }
@@ -117,12 +119,27 @@ struct Photon {
 };
 
 constexpr float TWO_PI = 6.283185307179586f;
+
+// (E / 60 keV)^(-k) of the three processes from one log2 and three exp2, and
+// the majorant mu_max = max_density * max_m sum_x a_x[m] f_x, and 1/mu_max
+// (infinity when mu_max is 0: every flight then leaves the box).
+__device__ __forceinline__ void cross_sections(const drr_mc_params_t &p, const float (*s_a)[DRR_MAX_MATERIALS],
+                                               float E, float &fpe, float &fc, float &fr, float &mumax,
+                                               float &imumax) {
+    const float l = __log2f(E * (1.0f / 60.0f));
+    fpe = exp2f(-p.k_pe * l);
+    fc = exp2f(-p.k_compton * l);
+    fr = exp2f(-p.k_rayleigh * l);
+    float smax = 0.0f;
+    for (int m = 0; m < p.n_materials; m++) smax = fmaxf(smax, s_a[0][m] * fpe + s_a[1][m] * fc + s_a[2][m] * fr);
+    mumax = p.max_density * smax;
+    imumax = mumax > 0.0f ? 1.0f / mumax : INFINITY;
+}
 constexpr float E_MIN = 5.0f;  // keV: lower energies are absorbed locally
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_mc(const float *__restrict__ dens, const uint8_t *__restrict__ lab,
-                                            int nx, int ny, int nz, drr_mc_params_t p, int64_t first,
+__global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int nx, int ny, int nz, drr_mc_params_t p, int64_t first,
                                             int64_t count, uint64_t seed, float *img, float *img2,
                                             unsigned long long *nsteps) {
     __shared__ float s_e[DRR_MAX_BINS], s_cdf[DRR_MAX_BINS];
@@ -146,7 +163,7 @@ __global__ __launch_bounds__(256) void k_mc(const float *__restrict__ dens, cons
     Photon ph;
     ph.alive = false;
     Ranecu rng{1, 1};
-    float fpe = 0.f, fc = 0.f, fr = 0.f, mumax = 0.f;
+    float fpe = 0.f, fc = 0.f, fr = 0.f, mumax = 0.f, imumax = INFINITY;
     float t_exit = 0.f;
     for (;;) {
         if (!ph.alive) {
@@ -191,19 +208,13 @@ __global__ __launch_bounds__(256) void k_mc(const float *__restrict__ dens, cons
             if (ph.inside) {
                 for (int a = 0; a < 3; a++) ph.x[a] += tmin * ph.d[a];
                 t_exit = tmax - tmin;
-                fpe = powf(ph.E / 60.0f, -p.k_pe);
-                fc = powf(ph.E / 60.0f, -p.k_compton);
-                fr = powf(ph.E / 60.0f, -p.k_rayleigh);
-                float smax = 0.0f;
-                for (int m = 0; m < p.n_materials; m++)
-                    smax = fmaxf(smax, s_a[0][m] * fpe + s_a[1][m] * fc + s_a[2][m] * fr);
-                mumax = p.max_density * smax;
+                cross_sections(p, s_a, ph.E, fpe, fc, fr, mumax, imumax);
             }
         }
         if (ph.inside) {
             // one Woodcock flight
             ++steps;
-            const float s = (mumax > 0.0f) ? -logf(ranecu(rng)) / mumax : INFINITY;
+            const float s = -__logf(ranecu(rng)) * imumax;
             if (s >= t_exit) {
                 for (int a = 0; a < 3; a++) ph.x[a] += t_exit * ph.d[a];
                 ph.inside = false;  // left the box: fly to the detector below
@@ -217,8 +228,9 @@ __global__ __launch_bounds__(256) void k_mc(const float *__restrict__ dens, cons
                     ix[a] = min(max((int)floorf(q + 0.5f), 0), nn[a] - 1);
                 }
                 const int64_t vi = ((int64_t)ix[2] * ny + ix[1]) * nx + ix[0];
-                const float rho = dens[vi];
-                const int m = lab[vi];
+                const uint2 rv = vox[vi];  // (density bits, label)
+                const float rho = __uint_as_float(rv.x);
+                const int m = (int)rv.y;
                 const float mpe = rho * s_a[0][m] * fpe;
                 const float mc = rho * s_a[1][m] * fc;
                 const float mr = rho * s_a[2][m] * fr;
@@ -247,16 +259,13 @@ __global__ __launch_bounds__(256) void k_mc(const float *__restrict__ dens, cons
                     for (int a = 0; a < 3; a++) {
                         const float q0 = A[4 * a] * ph.x[0] + A[4 * a + 1] * ph.x[1] + A[4 * a + 2] * ph.x[2] + A[4 * a + 3];
                         const float dq = A[4 * a] * ph.d[0] + A[4 * a + 1] * ph.d[1] + A[4 * a + 2] * ph.d[2];
-                        if (dq != 0.0f) tmax = fminf(tmax, fmaxf((-0.5f - q0) / dq, (nnf[a] - 0.5f - q0) / dq));
+                        if (dq != 0.0f) {
+                            const float idq = 1.0f / dq;
+                            tmax = fminf(tmax, fmaxf((-0.5f - q0) * idq, (nnf[a] - 0.5f - q0) * idq));
+                        }
                     }
                     t_exit = fmaxf(tmax, 0.0f);
-                    fpe = powf(ph.E / 60.0f, -p.k_pe);
-                    fc = powf(ph.E / 60.0f, -p.k_compton);
-                    fr = powf(ph.E / 60.0f, -p.k_rayleigh);
-                    float smax = 0.0f;
-                    for (int mm = 0; mm < p.n_materials; mm++)
-                        smax = fmaxf(smax, s_a[0][mm] * fpe + s_a[1][mm] * fc + s_a[2][mm] * fr);
-                    mumax = p.max_density * smax;
+                    cross_sections(p, s_a, ph.E, fpe, fc, fr, mumax, imumax);
                 }
             }
             continue;
@@ -316,12 +325,17 @@ extern "C" int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_
     if (!(pp.max_density > 0.0f) || pp.max_density < v->max_density) pp.max_density = v->max_density;
     int cus = 0;
     DRR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    // Every lane takes histories h, h + stride, ...: a grid larger than what is
+    // resident at once would leave its later blocks a full share of work to
+    // start only when the first ones finish, so the grid is capped there.
+    int per_cu = 0;
+    DRR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mc, 256, 0));
     int64_t blocks = (n_histories + 255) / 256;
-    const int64_t cap = (int64_t)cus * 16;
+    const int64_t cap = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
     if (blocks > cap) blocks = cap;
     hipStream_t st = (hipStream_t)stream;
     (void)hipEventRecord(ctx->ev0, st);
-    k_mc<<<(unsigned)blocks, 256, 0, st>>>(v->lin_dens, v->lin_lab, v->nx, v->ny, v->nz, pp, first_history,
+    k_mc<<<(unsigned)blocks, 256, 0, st>>>(v->lin_vox, v->nx, v->ny, v->nz, pp, first_history,
                                          n_histories, seed, image, image2,
                                          reinterpret_cast<unsigned long long *>(n_steps));
     hipError_t e = hipGetLastError();

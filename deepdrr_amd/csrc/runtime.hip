@@ -191,7 +191,8 @@ static int check_dims(int nx, int ny, int nz) {
     if (nx < 1 || ny < 1 || nz < 1) return drr_set_error(DRR_ERR_INVALID, "drr_volume: dimensions must be >= 1");
     if (nx > 16384 || ny > 16384 || nz > 16384)
         return drr_set_error(DRR_ERR_INVALID, "drr_volume: dimensions must be <= 16384");
-    // the march indexes cells (and 2 float4 per cell) with 32-bit integers
+    // the march forms cell indices in 32 bits, biased by 0x4B000000 (below
+    // 2^32 for fewer than 2^31 cells); byte offsets are 64-bit
     if ((int64_t)(nx + 1) * (ny + 1) * (nz + 1) >= ((int64_t)1 << 31))
         return drr_set_error(DRR_ERR_INVALID, "drr_volume: (nx+1)(ny+1)(nz+1) must be < 2^31 cells");
     return DRR_OK;
@@ -310,8 +311,7 @@ int drr_volume_destroy(drr_volume *vol) {
     if (!vol) return DRR_OK;
     DeviceGuard g(vol->ctx ? vol->ctx->device : 0);
     (void)hipFree(vol->cells);
-    (void)hipFree(vol->lin_dens);
-    (void)hipFree(vol->lin_lab);
+    (void)hipFree(vol->lin_vox);
     delete vol;
     return DRR_OK;
 }
@@ -332,31 +332,47 @@ __global__ void k_unexpand(const uint32_t *cells, int nx, int ny, int nz, float 
 
 }  // extern "C"
 
-__global__ void k_max_density(const float *d, int64_t n, unsigned int *out) {
+// Voxel-order (density bits, label) records from the cells, and the largest
+// density (each thread's maximum starts at 0, so it is non-negative, and
+// non-negative floats order like their bits).
+__global__ void k_voxels(const uint32_t *cells, int nx, int ny, int nz, uint2 *out, unsigned int *dmax) {
+    const int64_t n = (int64_t)nx * ny * nz;
     float m = 0.0f;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, d[i]);
-    atomicMax(out, __float_as_uint(m));  // non-negative floats order like their bits
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % nx);
+        const int y = (int)((i / nx) % ny);
+        const int z = (int)(i / ((int64_t)nx * ny));
+        const int64_t cell = ((int64_t)(z + 1) * (ny + 1) + (y + 1)) * (int64_t)(nx + 1) + (x + 1);
+        const uint32_t *rec = cells + DRR_CELL_WORDS * cell;  // corner 0 of the voxel's own cell
+        out[i] = make_uint2(rec[0], rec[8] & 0xfu);
+        m = fmaxf(m, __uint_as_float(rec[0]));
+    }
+    atomicMax(dmax, __float_as_uint(m));
 }
 
 int drr_volume_ensure_linear(drr_volume *vol) {
-    if (vol->lin_dens) return DRR_OK;
+    if (vol->lin_vox) return DRR_OK;
     const int64_t n = (int64_t)vol->nx * vol->ny * vol->nz;
-    DRR_HIP_CHECK(hipMalloc(&vol->lin_dens, sizeof(float) * n));
-    DRR_HIP_CHECK(hipMalloc(&vol->lin_lab, n));
-    k_unexpand<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, vol->lin_dens, vol->lin_lab);
-    DRR_HIP_CHECK(hipGetLastError());
+    uint2 *vox = nullptr;
     unsigned int *d_max = nullptr;
-    DRR_HIP_CHECK(hipMalloc(&d_max, sizeof(unsigned int)));
-    DRR_HIP_CHECK(hipMemset(d_max, 0, sizeof(unsigned int)));
-    k_max_density<<<1024, 256, 0, 0>>>(vol->lin_dens, n, d_max);
-    DRR_HIP_CHECK(hipGetLastError());
+    hipError_t e = hipMalloc(&vox, sizeof(uint2) * n);
+    if (e == hipSuccess) e = hipMalloc(&d_max, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemset(d_max, 0, sizeof(unsigned int));
+    if (e == hipSuccess) {
+        k_voxels<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, vox, d_max);
+        e = hipGetLastError();
+    }
     unsigned int bits = 0;
-    DRR_HIP_CHECK(hipMemcpy(&bits, d_max, sizeof(unsigned int), hipMemcpyDeviceToHost));
+    if (e == hipSuccess) e = hipMemcpy(&bits, d_max, sizeof(unsigned int), hipMemcpyDeviceToHost);
     (void)hipFree(d_max);
+    if (e != hipSuccess) {
+        (void)hipFree(vox);
+        return drr_hip_error(e, "voxel-order copy for the Monte-Carlo path");
+    }
     float mx;
     std::memcpy(&mx, &bits, sizeof(float));
+    vol->lin_vox = vox;
     vol->max_density = mx;
     return DRR_OK;
 }

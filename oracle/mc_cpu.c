@@ -10,8 +10,10 @@
  * scoring on the detector as [primary, single Compton, single Rayleigh,
  * multiple].  Random numbers: RANECU (L'Ecuyer 1988), one stream per history
  * seeded by SplitMix64 of (seed, history) -- the same streams the GPU uses, so
- * histories agree except where float library functions (logf, powf, cosf)
- * differ by an ulp.  Images accumulate in double per thread.
+ * histories agree except where the float functions differ by a few ulp (the
+ * GPU takes its log, powers, sine and cosine from the hardware log2, exp2,
+ * sin and cos) and flip a sampling decision.  Images accumulate in double per
+ * thread.
  */
 #include <math.h>
 #include <stdint.h>
