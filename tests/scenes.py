@@ -84,6 +84,24 @@ def multi_case(step=0.1, same_grid=False, atten=False, priorities=None) -> Case:
                      step=step, atten=atten, priorities=priorities)
 
 
+def many_case(n_vols, seed=0, atten=False, priorities=None) -> Case:
+    """A CT and n_vols - 1 uniform blocks of other grids, spacings and materials,
+    shifted so that they overlap it and each other (3 and 4 volumes run the 3-
+    and 4-volume marches)."""
+    rng = np.random.default_rng(seed)
+    vols = [phantoms.small_ct(20, seed)]
+    blocks = [((10, 12, 8), (1.5, 1.0, 2.0), 1.9, "bone"),
+              ((14, 9, 11), (1.2, 1.7, 1.0), 0.3, "lung"),
+              ((7, 16, 9), (2.0, 0.8, 1.6), 4.5, "titanium")]
+    for shape, spacing, dens, mat in blocks[:n_vols - 1]:
+        b = phantoms.uniform_volume(shape, spacing=spacing, density=dens, material=mat)
+        b.place_center(rng.uniform(-6.0, 6.0, 3).tolist())
+        vols.append(b)
+    dev = phantoms.ap_device(18, 14, 3.0)
+    return make_case(f"many{n_vols}_seed{seed}_atten{int(atten)}", vols, dev.get_camera_projection(),
+                     atten=atten, priorities=priorities)
+
+
 class GpuRunner:
     """Owns a drr_ctx and volume handles for one Case."""
 

@@ -45,6 +45,9 @@ CASES = [
     lambda: scenes.multi_case(),
     lambda: scenes.multi_case(same_grid=True),
     lambda: scenes.multi_case(atten=True, priorities=[0, 0]),
+    lambda: scenes.many_case(3, seed=0),
+    lambda: scenes.many_case(4, seed=1, priorities=[1, 0, 1, 0]),
+    lambda: scenes.many_case(4, seed=2, atten=True, priorities=[0, 0, 0, 0]),
 ]
 
 
