@@ -295,3 +295,64 @@ def test_clinical_size_band(gpu):
             assert np.array_equal(P.view(np.uint32), P_b[i].view(np.uint32)), f"view {i}"
     finally:
         r.close()
+
+
+MATERIAL_NAMES = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon"]
+
+
+def _materials_scene(seed, k):
+    """k materials (1 to 8) over a small anisotropic volume: 3x3x3 blocks of one
+    label (uniform cells, material switches) with 10% of the voxels relabelled at
+    random (mixed cells), two random poses, a random detector and step."""
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.volume import Volume
+
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(x) for x in rng.integers(4, 20, 3))
+    spacing = tuple(float(x) for x in rng.uniform(0.5, 2.0, 3))
+    dens = rng.uniform(0.0, 3.0, shape).astype(np.float32)
+    coarse = rng.integers(0, k, tuple((s + 2) // 3 for s in shape))
+    lab = np.repeat(np.repeat(np.repeat(coarse, 3, 0), 3, 1), 3, 2)[:shape[0], :shape[1], :shape[2]]
+    lab = np.where(rng.random(shape) < 0.1, rng.integers(0, k, shape), lab)
+    names = MATERIAL_NAMES[:k]
+    vol = Volume.from_parameters(dens, {m: lab == i for i, m in enumerate(names)},
+                                 origin=phantoms._centered_origin(shape, spacing), spacing=spacing,
+                                 anatomical_coordinate_system="LPS")
+    poses = phantoms.batch_poses(2, seed=seed, radius=0.3 * max(shape) * max(spacing), cone_deg=80.0)
+    w, h = (int(x) for x in rng.integers(4, 20, 2))
+    pix = float(rng.uniform(0.5, 4.0))
+    projs = [phantoms.pose_projection(w, h, pix, p, d, f) for p, d, f in poses]
+    step = float(rng.choice([0.05, 0.1, 0.37]))
+    case = scenes.make_case(f"materials{k}", [vol], projs[0], step=step, atten=(k % 3 == 0))
+    assert case.scene.n_materials == k
+    return case, vol, projs
+
+
+@pytest.mark.parametrize("k", range(1, 9))
+def test_material_counts(gpu, k):
+    """Every material count the marches are instantiated for (1 to 8; from 6 on
+    the marches run at a lower occupancy): bit-exact area densities against the
+    transcription, and batched images bit-identical to one view per launch with
+    both tile shapes."""
+    import torch
+
+    from deepdrr_amd.batch import BatchPlan
+
+    case, vol, projs = _materials_scene(40 + k, k)
+    print(f"case {case.name}", flush=True)
+    r = GpuRunner(case)
+    try:
+        _check(case, r.project(), _oracle(case))
+        views = [case.view] + [scenes.P.build_view([vol], [0], pr, 4080.0) for pr in projs[1:]]
+        singles = [r.project(v)[:2] for v in views]
+        for tuning in ((8, 8, 4, 8), (16, 4, 2, 4)):
+            plan = BatchPlan(r.ctx, [r.vols[0]], views, [0] * len(views), neglog=False, tuning=tuning)
+            plan.launch()
+            torch.cuda.synchronize()
+            I_b, P_b = plan.fetch("intensity"), plan.fetch("photon_prob")
+            plan.close()
+            for i, (I, P) in enumerate(singles):
+                assert np.array_equal(I.view(np.uint32), I_b[i].view(np.uint32)), f"view {i}, {tuning}"
+                assert np.array_equal(P.view(np.uint32), P_b[i].view(np.uint32)), f"view {i}, {tuning}"
+    finally:
+        r.close()
