@@ -108,7 +108,7 @@ def plain_views(torch, _abi, ctx, varr, views, dev, utils):
     return utils.neglog(np.stack(imgs)), kms
 
 
-def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=15.0):
+def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=40.0):
     """Oracle transcription on the host cores over a band of detector rows of the view."""
     from oracle import ctrans
 
@@ -129,7 +129,8 @@ def cpu_baseline(view, case_vols, scene, E, pdf, mu, budget_s=15.0):
         dt = time.perf_counter() - t
         if dt > budget_s / 4 or rows * 2 > H:
             break
-        rows = min(H, int(rows * max(2.0, budget_s / 4 / max(dt, 1e-3))))
+        # aim past the stopping time, so the last band is the one timed
+        rows = min(H, int(rows * max(2.0, 1.25 * budget_s / 4 / max(dt, 1e-3))))
     frac = rows / H
     return dict(value=frac / dt, unit="views/s", cores=threads, kind="port",
                 sample=f"{rows} of {H} detector rows of the AP 1536x1536 view ({frac:.4f} of the view), "
