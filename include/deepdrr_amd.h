@@ -197,7 +197,11 @@ int drr_add_noise(drr_ctx *ctx, const float *intensity, const float *photon_prob
  * source_world and pose_center_world are 3 host floats.  The per-ray march
  * is recomputed twice (area densities, then gradients); nothing is stored
  * per sample.  Boundary terms of the step count and of the volume entry/exit
- * (piecewise constant in the pose) are not differentiated. */
+ * (piecewise constant in the pose) are not differentiated.  With the scene's
+ * attenuate_outside, the air term outside the volume (minAlpha/step and
+ * (ray_length - maxAlpha)/step times the air constant, project_kernel.cu:
+ * 359-361, 558-560) is held constant as well, although maxAlpha moves with
+ * the pose. */
 int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t *view,
                 const float *grad_intensity, const float *grad_photon_prob,
                 const float *source_world, const float *pose_center_world,

@@ -93,8 +93,10 @@ class DenseDRR:
             self.views.append(dict(n=view.width * view.height, hw=(view.height, view.width),
                                    rays=torch.tensor(rays), x=torch.tensor(x), wt=torch.tensor(wts)))
 
-    def render(self, density, onehot, theta):
-        """density [ni,nj,nk] float64, onehot [M,ni,nj,nk] float64, theta [6] float64."""
+    def render(self, density, onehot, theta, extra_ad=None):
+        """density [ni,nj,nk] float64, onehot [M,ni,nj,nk] float64, theta [6] float64;
+        extra_ad: None or per view [M, H*W] area densities (g/cm^2) added to the
+        marched ones, held constant (the air outside the volume)."""
         R = rodrigues_t(theta[3:])
         T = torch.eye(4, dtype=torch.float64)
         T = T.clone()
@@ -104,7 +106,7 @@ class DenseDRR:
         ni, nj, nk = self.shape
         vol = torch.cat([density[None], onehot], 0).permute(0, 3, 2, 1)[None]  # (1, 1+M, nk, nj, ni)
         outs = []
-        for v in self.views:
+        for i, v in enumerate(self.views):
             p = (v["x"] @ ijk_from_w[:3, :3].T + ijk_from_w[:3, 3]) - 1.0
             n = torch.tensor([ni, nj, nk], dtype=torch.float64)
             g = 2.0 * p / (n - 1.0) - 1.0
@@ -114,6 +116,8 @@ class DenseDRR:
             contrib = (v["wt"] * rho)[None] * seg  # (M, S)
             ad = torch.zeros(self.M, v["n"], dtype=torch.float64).index_add(1, v["rays"], contrib)
             ad = ad * self.step / 10.0
+            if extra_ad is not None:
+                ad = ad + extra_ad[i]
             ex = torch.exp(-(self.mu @ ad))  # (B, R)
             I = ((self.E * self.pdf)[:, None] * ex).sum(0)
             outs.append(I.view(*v["hw"]))

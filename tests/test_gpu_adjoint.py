@@ -317,3 +317,73 @@ def test_adjoint_material_counts(gpu, k):
         assert err < 1e-4
     finally:
         dp.close()
+
+
+def test_adjoint_air_outside(gpu):
+    """The adjoint kernels with the air outside the volume attenuating (the
+    scene's attenuate_outside, reachable through the C ABI): gradients against
+    dense FP64 autograd of the same model, in which that outside air (read from
+    the forward's area densities) is a constant, as the adjoint treats it."""
+    import ctypes as C
+
+    from deepdrr_amd import _abi
+    from deepdrr_amd._abi_types import Job, Scene
+
+    dens, labels, W, projs, step = _scene(seed=7)
+    dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
+    try:
+        views = dp.views(projs, np.zeros(6))
+        H, Wd = views[0][0].height, views[0][0].width
+        d_kji = torch.tensor(np.ascontiguousarray(np.moveaxis(dens, [0, 1, 2], [2, 1, 0])), device=gpu)
+        h = dp.upload(d_kji)
+        varr = (C.c_void_p * 1)(h.value)
+        jobs = (Job * len(views))()
+        for i, (v, _) in enumerate(views):
+            jobs[i].view = v
+            jobs[i].volume_index = 0
+            jobs[i].out_offset = i * H * Wd
+
+        def area_densities():
+            I_j = torch.empty((len(views), H, Wd), device=gpu)
+            AD_j = torch.empty((len(views), len(MATS), H, Wd), device=gpu)
+            _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I_j),
+                      _abi.ptr(torch.empty_like(I_j)), _abi.ptr(AD_j), _abi.stream_ptr())
+            torch.cuda.synchronize()
+            return AD_j.cpu().numpy().astype(np.float64)
+
+        ad0 = area_densities()
+        sc = Scene(step=step, n_materials=len(MATS), attenuate_outside=1, air_index=MATS.index("air"))
+        _abi.call("drr_ctx_set_scene", dp.ctx, C.byref(sc))
+        ad1 = area_densities()
+        _abi.lib().drr_volume_destroy(h)
+        extra = ad1 - ad0
+        air = MATS.index("air")
+        assert np.all(extra[:, [m for m in range(len(MATS)) if m != air]] == 0.0)
+        # (minAlpha/step + (ray_length - maxAlpha)/step) * 0.1129 plus the steps
+        # before the volume: with unit-depth rays (ray_length ~ 1 mm) the
+        # reference's suffix is negative, so only its presence is asserted
+        assert np.abs(extra[:, air]).max() > 0.0
+        extra_t = [torch.tensor(e.reshape(len(MATS), H * Wd)) for e in extra]
+
+        rng = np.random.default_rng(3)
+        gI = rng.standard_normal((len(projs), H, Wd)).astype(np.float32)
+        d_t = torch.tensor(dens, device=gpu, requires_grad=True)
+        th_t = torch.zeros(6, device=gpu, requires_grad=True)
+        print("adjoint with air outside the volume", flush=True)
+        I = dp(d_t, th_t, projs)
+        (I * torch.tensor(gI, device=gpu)).sum().backward()
+        gD = d_t.grad.detach().cpu().numpy().astype(np.float64)
+        gth = th_t.grad.detach().cpu().numpy().astype(np.float64)
+        ref = _dense(dp, projs)
+        d64 = torch.tensor(dens.astype(np.float64), requires_grad=True)
+        th64 = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+        I64 = ref.render(d64, _onehot(labels), th64, extra_ad=extra_t)
+        np.testing.assert_allclose(I.detach().cpu().numpy(), I64.detach().numpy(), rtol=2e-5)
+        (I64 * torch.tensor(gI, dtype=torch.float64)).sum().backward()
+        rel = np.linalg.norm(gD - d64.grad.numpy()) / np.linalg.norm(d64.grad.numpy())
+        err = np.linalg.norm(gth - th64.grad.numpy()) / np.linalg.norm(th64.grad.numpy())
+        print(f"air outside: density grad rel L2 {rel:.2e}, pose grad rel L2 {err:.2e}")
+        assert rel < 1e-4
+        assert err < 1e-4
+    finally:
+        dp.close()
