@@ -489,17 +489,13 @@ struct AdjArgs {
     float *gPose;     // [6] or NULL
 };
 
-__device__ __forceinline__ int xcd_remap_adj(int bid, int nb) {
-    const int xcd = bid & 7, idx = bid >> 3;
-    const int per = nb >> 3, rem = nb & 7;
-    return xcd * per + min(xcd, rem) + idx;
-}
-
 template <int M, bool ATTEN, bool POSE>
 __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step, int air,
                                                     AdjArgs a, float swx, float swy, float swz, int tiles16_x) {
     __shared__ double pose_slots[POSE ? 6 * 256 : 1];
-    const int lb = xcd_remap_adj(blockIdx.x, gridDim.x);
+    // tiles in row-major order, dispatched round-robin over the XCDs, so
+    // every XCD gets a share of every band of rows (of ray lengths)
+    const int lb = blockIdx.x;
     const int bx = lb % tiles16_x, by = lb / tiles16_x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int udx, vdx;
