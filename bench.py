@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline and plain path only")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint measurement (192^3, 64 views of 256^2)")
+    ap.add_argument("--mc-only", action="store_true",
+                    help="only the Monte-Carlo measurement (--mc-histories on the 512x512x400 phantom)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprof)")
     args = ap.parse_args()
@@ -308,6 +310,19 @@ def main():
         import torch
 
         print(json.dumps({"adjoint": measure_adjoint(torch)}), flush=True)
+        return
+    if args.mc_only:
+        import torch
+
+        from deepdrr_amd import _abi, workload
+
+        materials = sorted(workload.CLINICAL_CLASSES)
+        ctx, _ = make_ctx(_abi, materials, args.step_mm)
+        vh, _ = workload.clinical_volume_device(ctx, materials)
+        mc = measure_mc(torch, _abi, ctx, vh, workload.clinical_geometry_volume(), args.mc_histories)
+        _abi.lib().drr_volume_destroy(vh)
+        _abi.lib().drr_ctx_destroy(ctx)
+        print(json.dumps({"monte_carlo": mc}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
