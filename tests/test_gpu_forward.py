@@ -356,3 +356,35 @@ def test_material_counts(gpu, k):
                 assert np.array_equal(P.view(np.uint32), P_b[i].view(np.uint32)), f"view {i}, {tuning}"
     finally:
         r.close()
+
+
+def test_projector_collected_energy_and_upper_bound(gpu):
+    """Projector(collected_energy=True): intensity x solid angle x photon_count /
+    mean(solid angle) / (pixel_x pixel_y) with pixel = SDD / focal length
+    (projector.py:833-853), the solid angles from the GPU checked against the
+    oracle's; and intensity_upper_bound clips the images as project() does."""
+    from oracle import ctrans
+
+    from deepdrr_amd import Projector, phantoms
+    from deepdrr_amd.spectral_data import synthetic_120kv
+
+    vol = phantoms.small_ct(24)
+    dev = phantoms.ap_device(20, 16, 2.0)
+    cam = dev.get_camera_projection()
+    kw = dict(device=dev, step=0.1, spectrum=synthetic_120kv(), neglog=False)
+    with Projector(vol, **kw) as proj:
+        I = proj(cam)
+        view = proj.view_for(cam)
+    with Projector(vol, collected_energy=True, photon_count=5000, **kw) as proj:
+        E = proj(cam)
+        sdd = proj.source_to_detector_distance
+    assert sdd == 1020.0
+    sa = ctrans.solid_angle(view.world_from_index[:], view.width, view.height)
+    px = sdd / cam.index_from_camera2d.fx
+    py = sdd / cam.index_from_camera2d.fy
+    np.testing.assert_allclose(E, I.astype(np.float64) * sa * 5000 / np.average(sa) / (px * py), rtol=1e-5)
+    ub = float(np.percentile(I, 60))
+    with Projector(vol, intensity_upper_bound=ub, **kw) as proj:
+        C = proj(cam)
+    assert np.array_equal(C, np.clip(I, None, ub))
+    assert (C < I).any()
