@@ -531,8 +531,8 @@ __global__ void k_pose_finish(const double *acc, float *gPose) {
 // volume directly, so results do not depend on the geometry, only the
 // number of global atomics does.
 constexpr int ADJ_TILE = 16;
-constexpr int ADJ_SLAB = 11;
-constexpr int ADJ_BOX = 24;
+constexpr int ADJ_SLAB = 15;
+constexpr int ADJ_BOX = 20;
 constexpr int ADJ_BOX_CELLS = (ADJ_SLAB + 1) * ADJ_BOX * ADJ_BOX;
 
 // World ray through the detector point (fu, fv) (pixel-edge coordinates).
