@@ -202,15 +202,18 @@ def test_adjoint_fd_single_material_smooth(gpu):
         dp.close()
 
 
-def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu):
+@pytest.mark.parametrize("n,det,n_views,seed", [(18, 14, 4, 5), (40, 32, 3, 9)])
+def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu, n, det, n_views, seed):
     """drr_adjoint_jobs (one persistent launch) == sum over views of drr_adjoint,
-    up to float atomic ordering; drr_project_jobs == drr_project bit for bit."""
+    up to float atomic ordering; drr_project_jobs == drr_project bit for bit.
+    The 40^3 case is wider than the LDS box, so oblique beams leave it and their
+    outer corners take the direct global path."""
     import ctypes as C
 
     from deepdrr_amd import _abi
     from deepdrr_amd._abi_types import Job
 
-    dens, labels, W, projs, step = _scene(n=18, n_views=4, det=14, seed=5)
+    dens, labels, W, projs, step = _scene(n=n, n_views=n_views, det=det, seed=seed)
     dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
     try:
         views = dp.views(projs, np.zeros(6))
