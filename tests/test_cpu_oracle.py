@@ -1,7 +1,8 @@
 """Pins for the CPU oracle (C transcription + FP64 restatement).
 
-The reference's own goldens cannot run offline (SURVEY: CT-chest.nrrd download,
-killeengeo, CUDA), so the oracle is pinned by what the mathematics fixes:
+The reference's own goldens (tests/reference/*.png) cannot run offline: they need
+the CT-chest.nrrd download, killeengeo and CUDA.  So the oracle is pinned by what
+the mathematics fixes:
 closed forms (empty volume, homogeneous slab Beer-Lambert), exact invariants
 (density scaling by 2 doubles area densities bit-exactly), the two
 independent restatements agreeing on random tiny scenes, and the reference's
