@@ -325,7 +325,8 @@ def main():
         print(json.dumps({"monte_carlo": mc}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+        print(json.dumps({"error": "this engine runs on one GPU; the views would shard over GPUs, but no "
+                                   "multi-GPU path is implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
