@@ -63,23 +63,24 @@ PMC_SUMMARY = os.path.join("profiles", "k_batch_march_clinical_1536_pmc.md")
 def profiled_counters(views_per_launch):
     """Counter figures of the batched march on this workload, read from the
     committed rocprofv3 summary: HBM bytes per launch (FETCH_SIZE + WRITE_SIZE,
-    each scaled by a copy-kernel calibration) and the VALU issue figures
-    (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2, SQ_CYCLES).  Returns (bytes, schedule
-    the counters were taken with, valu dict), Nones when the summary is absent
-    or was taken at another batch size."""
+    each scaled by a copy-kernel calibration), the VALU issue figures
+    (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2, SQ_CYCLES) and the L2 hit rate
+    (TCC_HIT / (TCC_HIT + TCC_MISS)).  Returns (bytes, schedule the counters were
+    taken with, valu dict, L2 hit rate), Nones when the summary is absent or was
+    taken at another batch size."""
     path = os.path.join(ROOT, PMC_SUMMARY)
     if not os.path.exists(path):
-        return None, None, None
+        return None, None, None, None
     vals = {}
     keys = ("traffic_bytes_per_launch", "views_per_launch", "tuning", "valu_issue_frac",
-            "valu_dual_issue_share", "valu_lane_instructions_per_sample")
+            "valu_dual_issue_share", "valu_lane_instructions_per_sample", "l2_hit_rate")
     for ln in open(path):
         ln = ln.strip()
         for key in keys:
             if ln.startswith(key + ":"):
                 vals[key] = ln.split(":", 1)[1].strip()
     if "traffic_bytes_per_launch" not in vals or int(vals.get("views_per_launch", -1)) != views_per_launch:
-        return None, None, None
+        return None, None, None, None
     valu = None
     if "valu_issue_frac" in vals:
         valu = {"issue_frac": float(vals["valu_issue_frac"]),
@@ -88,7 +89,8 @@ def profiled_counters(views_per_launch):
                 "note": "VALU issue quad-cycles used / available over the launch "
                         "((SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2) / (SQ_CYCLES x 8)); the kernel's binding "
                         "resource", "source": PMC_SUMMARY}
-    return float(vals["traffic_bytes_per_launch"]), [int(x) for x in vals.get("tuning", "").split()], valu
+    l2 = float(vals["l2_hit_rate"]) if "l2_hit_rate" in vals else None
+    return float(vals["traffic_bytes_per_launch"]), [int(x) for x in vals.get("tuning", "").split()], valu, l2
 
 
 def plain_views(torch, _abi, ctx, varr, views, dev, utils):
@@ -390,7 +392,7 @@ def main():
     value = B * args.steps / dt
     kernel_ms = float(np.mean(kms))
     achieved = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
-    traffic, traffic_tuning, valu = profiled_counters(B)
+    traffic, traffic_tuning, valu, l2_hit = profiled_counters(B)
     log(f"batched: {value:.2f} views/s, march {kernel_ms / B:.2f} ms/view")
 
     result = dict(
@@ -410,7 +412,7 @@ def main():
                   "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                   "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated, schedule "
                                      f"{traffic_tuning} ({PMC_SUMMARY})" if traffic is not None else None),
-                  "valu": valu,
+                  "valu": valu, "l2_hit_rate": l2_hit,
                   "kernel": f"k_batch_march<{len(materials)},false,{plan.tuning[0]}>",
                   "kernel_ms": kernel_ms, "samples_per_launch": samples_per_launch,
                   "bytes_per_sample": workload.BYTES_PER_SAMPLE,
