@@ -420,3 +420,33 @@ def test_longest_dimension(gpu, direction):
         _check(case, g, o)
     finally:
         r.close()
+
+
+def test_widest_detector(gpu):
+    """A detector 65536 pixels wide (the largest a view may have) and 3 rows:
+    bit-exact area densities, and batched images (both tile shapes, ragged
+    last tiles) bit-identical to the single view."""
+    import torch
+
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.batch import BatchPlan
+
+    vol = phantoms.small_ct(24)
+    proj = phantoms.pose_projection(65536, 3, 0.002, [0.0, 0.0, 0.0], [0.0, 1.0, 0.0], 0.5)
+    case = scenes.make_case("widest_detector", [vol], proj)
+    print("case widest_detector", flush=True)
+    r = GpuRunner(case)
+    try:
+        g = r.project()
+        o = _oracle(case)
+        assert (o[2] > 0).sum() > 1000
+        _check(case, g, o)
+        for tuning in ((8, 8, 4, 8), (16, 4, 2, 4)):
+            plan = BatchPlan(r.ctx, [r.vols[0]], [case.view], [0], neglog=False, tuning=tuning)
+            plan.launch()
+            torch.cuda.synchronize()
+            I_b = plan.fetch("intensity")[0]
+            plan.close()
+            assert np.array_equal(g[0].view(np.uint32), I_b.view(np.uint32)), tuning
+    finally:
+        r.close()
