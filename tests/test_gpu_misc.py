@@ -117,3 +117,52 @@ def test_non_finite_view_rejected(gpu):
             assert rc != 0 and "finite" in _abi.last_error()
     finally:
         r.close()
+
+
+def test_job_view_and_plan_errors(gpu):
+    """Invalid jobs, views and batch plans are refused on the host, before any
+    launch, with a message (the reference's Projector raises for its own
+    invalid inputs; the C ABI returns a status and drr_last_error)."""
+    from deepdrr_amd._abi_types import Job
+    from deepdrr_amd.batch import BatchPlan
+    from tests import scenes
+
+    case = scenes.small_case(8, (6, 5))
+    r = scenes.GpuRunner(case)
+    try:
+        out = torch.empty(256, device=gpu)
+
+        def project_jobs(jobs, n):
+            return _abi.lib().drr_project_jobs(r.ctx, r.varr, 1, jobs, n, _abi.ptr(out), _abi.ptr(out), None,
+                                               _abi.stream_ptr())
+
+        J = (Job * 1)()
+        J[0].view = case.view
+        J[0].volume_index = 0
+        J[0].out_offset = 0
+        assert project_jobs(J, 0) != 0 and "jobs" in _abi.last_error()
+        J[0].volume_index = 1
+        assert project_jobs(J, 1) != 0 and "volume_index" in _abi.last_error()
+        J[0].volume_index = 0
+        J[0].out_offset = -1
+        assert project_jobs(J, 1) != 0 and "offset" in _abi.last_error()
+        J[0].out_offset = 0
+        J[0].view.n_volumes = 2
+        assert project_jobs(J, 1) != 0 and "exactly one volume" in _abi.last_error()
+        for field, val in (("n_volumes", 0), ("n_volumes", 5), ("width", 0), ("height", 65537)):
+            v = type(case.view).from_buffer_copy(case.view)
+            setattr(v, field, val)
+            rc = _abi.lib().drr_project(r.ctx, r.varr, C.byref(v), _abi.ptr(out), _abi.ptr(out), None, None,
+                                        _abi.stream_ptr())
+            assert rc != 0, (field, val)
+        with pytest.raises(ValueError):
+            BatchPlan(r.ctx, [r.vols[0]], [], [])
+        with pytest.raises(RuntimeError, match="tile"):
+            BatchPlan(r.ctx, [r.vols[0]], [case.view], [0], neglog=False, tuning=(12, 8, 4, 8))
+        with pytest.raises(RuntimeError, match="waves_per_block"):
+            BatchPlan(r.ctx, [r.vols[0]], [case.view], [0], neglog=False, tuning=(8, 8, 5, 8))
+        # the context still works after the refusals
+        g = r.project()
+        assert np.all(np.isfinite(g[0]))
+    finally:
+        r.close()
