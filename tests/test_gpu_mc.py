@@ -89,3 +89,46 @@ def test_projector_with_scatter(gpu):
         withs = p1()
     assert withs.shape == base.shape
     assert np.all(withs >= base - 1e-3) and np.mean(withs - base) > 0
+
+
+def test_mc_parameter_errors(gpu):
+    """Invalid transport parameters are refused on the host with a message, and
+    zero histories is a no-op."""
+    from deepdrr_amd.spectral_data import synthetic_120kv
+
+    vol = _phantom(8)
+    mats = sorted(vol.materials[0].keys())
+    dens, labs = P.volume_arrays(vol, mats)
+    proj = phantoms.ap_device(8, 8, 4.0, sdd=600.0).get_camera_projection()
+    ctx = C.c_void_p()
+    _abi.call("drr_ctx_create", 0, C.byref(ctx))
+    try:
+        sc = Scene(step=0.1, n_materials=len(mats), attenuate_outside=0, air_index=0)
+        _abi.call("drr_ctx_set_scene", ctx, C.byref(sc))
+        h = C.c_void_p()
+        nz, ny, nx = dens.shape
+        _abi.call("drr_volume_create", ctx, dens.ctypes.data, labs.ctypes.data, nx, ny, nz, 0, C.byref(h))
+        img = torch.zeros((4, 8, 8), device=gpu)
+
+        def run(p, n=1000):
+            return _abi.lib().drr_mc_scatter(ctx, h, C.byref(p), 0, n, C.c_uint64(1), _abi.ptr(img), None, None,
+                                             _abi.stream_ptr())
+
+        def params(**kw):
+            p = mc.make_params(mats, synthetic_120kv(), proj, 600.0, vol.world_from_IJK.data)
+            for k, v in kw.items():
+                setattr(p, k, v)
+            return p
+
+        assert run(params(cone_cos=1.0)) != 0 and "cone_cos" in _abi.last_error()
+        assert run(params(max_interactions=0)) != 0 and "max_interactions" in _abi.last_error()
+        assert run(params(n_materials=len(mats) - 1)) != 0 and "label" in _abi.last_error()
+        bad_cdf = params()
+        bad_cdf.cdf[bad_cdf.n_bins - 1] = 0.5
+        assert run(bad_cdf) != 0 and "cdf" in _abi.last_error()
+        assert run(params(), n=0) == 0
+        torch.cuda.synchronize()
+        assert float(img.abs().sum()) == 0.0
+        _abi.lib().drr_volume_destroy(h)
+    finally:
+        _abi.lib().drr_ctx_destroy(ctx)
