@@ -774,10 +774,11 @@ extern "C" int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t
                            const float *grad_intensity, const float *grad_photon_prob,
                            const float *source_world, const float *pose_center_world,
                            float *grad_density, float *grad_pose, void *stream) {
+    // before drr_check_view, which reads n_volumes entries of the volume array
+    if (view && view->n_volumes != 1) return drr_set_error(DRR_ERR_UNSUPPORTED, "drr_adjoint: one volume per view");
     const drr_volume *vols[1] = {vol};
     int rc = drr_check_view(ctx, vols, view);
     if (rc) return rc;
-    if (view->n_volumes != 1) return drr_set_error(DRR_ERR_UNSUPPORTED, "drr_adjoint: one volume per view");
     rc = check_adj(ctx, vol, grad_intensity, grad_photon_prob, grad_density, grad_pose, source_world,
                    pose_center_world);
     if (rc) return rc;

@@ -390,3 +390,51 @@ def test_adjoint_air_outside(gpu):
         assert err < 1e-4
     finally:
         dp.close()
+
+
+def test_adjoint_errors(gpu):
+    """Invalid adjoint calls are refused on the host with a message, before any
+    launch, and leave the gradient buffers untouched."""
+    import ctypes as C
+
+    from deepdrr_amd import _abi
+    from deepdrr_amd._abi_types import Job
+
+    dens, labels, W, projs, step = _scene(n=12, n_views=1, det=8)
+    dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
+    try:
+        (v, s), = dp.views(projs, np.zeros(6))
+        d_kji = torch.tensor(np.ascontiguousarray(np.moveaxis(dens, [0, 1, 2], [2, 1, 0])), device=gpu)
+        h = dp.upload(d_kji)
+        gI = torch.zeros((v.height, v.width), device=gpu)
+        gD = torch.zeros_like(d_kji)
+        gT = torch.zeros(6, device=gpu)
+        c = np.asarray(dp.center, np.float32)
+        lib, st = _abi.lib(), _abi.stream_ptr()
+        J = (Job * 1)()
+        J[0].view = v
+        J[0].volume_index = 0
+        J[0].out_offset = 0
+
+        def jobs(n, gi=gI, gd=gD, gt=gT, src=s.ctypes.data):
+            return lib.drr_adjoint_jobs(dp.ctx, h, J, n, None if gi is None else _abi.ptr(gi), None, src,
+                                        c.ctypes.data, None, None if gd is None else _abi.ptr(gd),
+                                        None if gt is None else _abi.ptr(gt), st)
+
+        assert jobs(0) != 0 and "no jobs" in _abi.last_error()
+        J[0].volume_index = 1
+        assert jobs(1) != 0 and "volume 0" in _abi.last_error()
+        J[0].volume_index = 0
+        assert jobs(1, gd=None, gt=None) != 0 and "neither" in _abi.last_error()
+        assert jobs(1, gi=None) != 0 and "no output gradient" in _abi.last_error()
+        assert jobs(1, src=None) != 0 and "source_world" in _abi.last_error()
+        v2 = type(v).from_buffer_copy(v)
+        v2.n_volumes = 2
+        rc = lib.drr_adjoint(dp.ctx, h, C.byref(v2), _abi.ptr(gI), None, s.ctypes.data, c.ctypes.data,
+                             _abi.ptr(gD), None, st)
+        assert rc != 0 and "one volume" in _abi.last_error()
+        torch.cuda.synchronize()
+        assert float(gD.abs().sum()) == 0.0 and float(gT.abs().sum()) == 0.0
+        lib.drr_volume_destroy(h)
+    finally:
+        dp.close()
