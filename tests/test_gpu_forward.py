@@ -251,7 +251,7 @@ def _random_scene(seed):
     return scenes.make_case(f"random{seed}", [vol], proj, step=step, atten=bool(rng.random() < 0.4))
 
 
-@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("seed", range(40))
 def test_random_scenes(gpu, seed):
     """Bit-exact area densities on random anisotropic volumes, poses, detectors
     and steps (the cell loads, the label identity, the FP32 cell index, the
