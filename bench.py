@@ -94,9 +94,13 @@ def profiled_counters(views_per_launch):
 
 
 def plain_views(torch, _abi, ctx, varr, views, dev, utils):
-    """The reference's per-view flow; returns images [N,H,W] and kernel ms per view."""
+    """The reference's per-view flow (projector.py:655-707): per view, the host
+    geometry (``views`` holds callables that build the drr_view_t from the
+    camera projection), one launch, sync, D2H of intensity and photon_prob, and
+    NumPy neglog; returns images [N,H,W] and kernel ms per view."""
     imgs, kms = [], []
-    for view in views:
+    for make_view in views:
+        view = make_view()
         I = torch.empty((view.height, view.width), dtype=torch.float32, device=dev)
         P = torch.empty_like(I)
         _abi.call("drr_project", ctx, varr, C.byref(view), _abi.ptr(I), _abi.ptr(P), None, None,
@@ -349,14 +353,17 @@ def main():
     log(f"phantom + GPU ingestion {time.perf_counter() - t:.1f} s")
     varr = (C.c_void_p * 1)(vh.value)
     geom = workload.clinical_geometry_volume()
-    ap_view = P.build_view([geom], [0], phantoms.ap_device(1536, 1536, 0.194).get_camera_projection(),
-                           4 * 1020.0)
+    ap_proj = phantoms.ap_device(1536, 1536, 0.194).get_camera_projection()
+    ap_view = P.build_view([geom], [0], ap_proj, 4 * 1020.0)
 
     # ---- plain path: the reference's flow, one view per launch ----------------
-    plain_views(torch, _abi, ctx, varr, [ap_view], dev, utils)
+    def ap_view_host():
+        return P.build_view([geom], [0], ap_proj, 4 * 1020.0)
+
+    plain_views(torch, _abi, ctx, varr, [ap_view_host], dev, utils)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    _, kms_plain = plain_views(torch, _abi, ctx, varr, [ap_view] * args.plain_views, dev, utils)
+    _, kms_plain = plain_views(torch, _abi, ctx, varr, [ap_view_host] * args.plain_views, dev, utils)
     torch.cuda.synchronize()
     plain_vps = args.plain_views / (time.perf_counter() - t)
     plain_kernel_ms = float(np.mean(kms_plain))
