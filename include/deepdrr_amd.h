@@ -118,7 +118,9 @@ int drr_volume_download(drr_volume *vol, float *density, uint8_t *labels);
 /* ---- forward projection ------------------------------------------------ */
 
 /* One view, one launch: the reference's projectKernel (project_kernel.cu:135-650).
- * vols[v] pairs with view->vol[v].  Outputs are device float32 [height][width]
+ * vols holds view->n_volumes handles (read up to that count once n_volumes is
+ * checked to be in [1, 4]); vols[v] pairs with view->vol[v].  Outputs are device
+ * float32 [height][width]
  * (row-major; the reference's column-major buffer after its host swapaxes,
  * projector.py:786-792).  area_density (optional, may be NULL) receives the
  * per-material area densities in g/cm^2 as [n_materials][height][width].
