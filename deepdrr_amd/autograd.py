@@ -136,6 +136,31 @@ class _DRRFunction:
     pass
 
 
+class _VolumeHandle:
+    """Owns one device volume of the engine.  The handle is destroyed when
+    ``release`` is called or, at the latest, when the owner is garbage
+    collected (an autograd graph dropped without a backward pass frees it)."""
+
+    def __init__(self, handle: C.c_void_p):
+        import weakref
+
+        self.handle = handle
+        self._fin = weakref.finalize(self, _destroy_volume, handle.value)
+
+    def release(self) -> None:
+        self._fin()
+        self.handle = None
+
+    @property
+    def alive(self) -> bool:
+        return self._fin.alive
+
+
+def _destroy_volume(value) -> None:
+    if value:
+        _abi.lib().drr_volume_destroy(C.c_void_p(value))
+
+
 def _make_function():
     import torch
 
@@ -153,11 +178,11 @@ def _make_function():
                 jobs[i].view = view
                 jobs[i].volume_index = 0
                 jobs[i].out_offset = i * H * W
-            h = dp.upload(dens_kji)
-            varr = (C.c_void_p * 1)(h.value)
+            vol = _VolumeHandle(dp.upload(dens_kji))
+            varr = (C.c_void_p * 1)(vol.handle.value)
             I = torch.empty((len(views), H, W), dtype=torch.float32, device=dp.device)
             need_grad = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
-            # the area densities of the forward march, reused by the backward pass
+            # the area densities of the forward march, reused by the first backward pass
             AD = (torch.empty((len(views), len(dp.materials), H, W), dtype=torch.float32, device=dp.device)
                   if need_grad else None)
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
@@ -165,16 +190,18 @@ def _make_function():
                 _abi.call("drr_project_jobs", dp.ctx, varr, 1, jobs, len(views), _abi.ptr(I), None,
                           _abi.ptr(AD), stream)
             except Exception:
-                _abi.lib().drr_volume_destroy(h)
+                vol.release()
                 raise
+            if not need_grad:
+                vol.release()
+                vol = None
+            else:
+                ctx.save_for_backward(density)
             ctx.AD = AD
             ctx.dp = dp
             ctx.jobs = jobs
             ctx.sources = np.ascontiguousarray(np.stack([src for _, src in views]), np.float32)
-            ctx.vol = h
-            if not need_grad:
-                _abi.lib().drr_volume_destroy(h)
-                ctx.vol = None
+            ctx.vol = vol
             ctx.shape = density.shape
             ctx.need_pose = theta.requires_grad
             ctx.need_density = density.requires_grad
@@ -185,6 +212,12 @@ def _make_function():
             dp = ctx.dp
             grad_out = grad_out.contiguous().float()
             ni, nj, nk = ctx.shape
+            vol = ctx.vol
+            if vol is None or not vol.alive:
+                # a second backward through a retained graph: the first one
+                # released the device volume; upload the saved density again
+                (density,) = ctx.saved_tensors
+                vol = _VolumeHandle(dp.upload(density.detach().permute(2, 1, 0).contiguous().float()))
             # pose-only backward (fixed density, e.g. registration) scatters nothing
             gD = (torch.zeros((nk, nj, ni), dtype=torch.float32, device=dp.device)
                   if ctx.need_density else None)
@@ -192,11 +225,14 @@ def _make_function():
             stream = _abi.stream_ptr(torch.cuda.current_stream(dp.device))
             c = np.asarray(dp.center, np.float32)
             try:
-                _abi.call("drr_adjoint_jobs", dp.ctx, ctx.vol, ctx.jobs, len(ctx.jobs), _abi.ptr(grad_out),
+                _abi.call("drr_adjoint_jobs", dp.ctx, vol.handle, ctx.jobs, len(ctx.jobs), _abi.ptr(grad_out),
                           None, ctx.sources.ctypes.data, c.ctypes.data, _abi.ptr(ctx.AD), _abi.ptr(gD),
                           _abi.ptr(gP), stream)
             finally:
-                _abi.lib().drr_volume_destroy(ctx.vol)
+                # the device volume (40 B per cell) and the saved area densities
+                # are not kept between backward passes; a retained graph's next
+                # backward re-uploads the density and re-marches the forward
+                vol.release()
                 ctx.vol = None
                 ctx.AD = None
             grad_theta = gP if ctx.need_pose else None

@@ -23,6 +23,7 @@
 
 struct drr_volume_s {
     drr_ctx *ctx = nullptr;
+    int device = 0;  // HIP device of the allocation (outlives the context)
     int nx = 0, ny = 0, nz = 0;
     int64_t n_cells = 0;
     uint32_t *cells = nullptr;  // DRR_CELL_WORDS 32-bit words per cell

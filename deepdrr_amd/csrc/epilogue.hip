@@ -3,6 +3,8 @@
 // offsets, as used by the batched path and drr_neglog.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "drr_internal.h"
 #include "epilogue.h"
 
@@ -118,17 +120,16 @@ extern "C" int drr_neglog(drr_ctx *ctx, const float *in, float *out, int n_image
     hipStream_t st = (hipStream_t)stream;
     ImgSeg *segs = nullptr;
     unsigned int *keys = nullptr;
-    DRR_HIP_CHECK(hipMalloc(&segs, sizeof(ImgSeg) * n_images));
-    DRR_HIP_CHECK(hipMalloc(&keys, sizeof(unsigned int) * 4 * n_images));
-    ImgSeg *h = new ImgSeg[n_images];
+    std::vector<ImgSeg> h(n_images);
     for (int i = 0; i < n_images; i++) h[i] = ImgSeg{(int64_t)i * npix, npix};
-    hipError_t e = hipMemcpyAsync(segs, h, sizeof(ImgSeg) * n_images, hipMemcpyHostToDevice, st);
     int rc = DRR_OK;
-    if (e != hipSuccess) rc = drr_hip_error(e, "drr_neglog copy");
+    hipError_t e = hipMalloc(&segs, sizeof(ImgSeg) * n_images);
+    if (e == hipSuccess) e = hipMalloc(&keys, sizeof(unsigned int) * 4 * n_images);
+    if (e == hipSuccess) e = hipMemcpyAsync(segs, h.data(), sizeof(ImgSeg) * n_images, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) rc = drr_hip_error(e, "drr_neglog setup");
     if (rc == DRR_OK) rc = drr_enqueue_neglog(in, out, segs, n_images, npix, keys, epsilon, st);
     e = hipStreamSynchronize(st);
     if (rc == DRR_OK && e != hipSuccess) rc = drr_hip_error(e, "drr_neglog");
-    delete[] h;
     (void)hipFree(segs);
     (void)hipFree(keys);
     return rc;

@@ -66,7 +66,8 @@ __device__ float poisson(Stream &s, float lam) {
         }
         return (float)k;
     }
-    const float slam = sqrtf(lam), loglam = logf(lam);
+    const float slam = sqrtf(lam);
+    const double loglam_d = log((double)lam);
     const float b = 0.931f + 2.53f * slam;
     const float a = -0.059f + 0.02483f * b;
     const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
@@ -78,8 +79,10 @@ __device__ float poisson(Stream &s, float lam) {
         const float k = floorf((2.0f * a / us + b) * U + lam + 0.43f);
         if (us >= 0.07f && V <= vr) return k;
         if (k < 0.0f || (us < 0.013f && V > us)) continue;
-        if (logf(V) + logf(invalpha) - logf(a / (us * us) + b) <= -lam + k * loglam - lgammaf(k + 1.0f))
-            return k;
+        // log of the Poisson pmf at k: its three terms reach ~1e7 at lam ~ 1e6
+        // and cancel, so they are summed in FP64
+        const double logpmf = -(double)lam + (double)k * (double)loglam_d - lgamma((double)k + 1.0);
+        if ((double)(logf(V) + logf(invalpha) - logf(a / (us * us) + b)) <= logpmf) return k;
     }
     return roundf(lam);
 }

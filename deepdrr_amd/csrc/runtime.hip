@@ -41,6 +41,23 @@ struct DeviceGuard {
     }
 };
 
+// Device buffer freed at scope exit unless released: every early return of
+// an entry point frees what it allocated so far.
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+};
+
 }  // namespace
 
 extern "C" {
@@ -172,6 +189,7 @@ __global__ void k_hu_ingest(const int16_t *hu, int64_t n, int n_classes, const f
 static int volume_alloc(drr_ctx *ctx, int nx, int ny, int nz, drr_volume **out) {
     drr_volume *v = new drr_volume();
     v->ctx = ctx;
+    v->device = ctx->device;
     v->nx = nx;
     v->ny = ny;
     v->nz = nz;
@@ -211,28 +229,28 @@ int drr_volume_create(drr_ctx *ctx, const float *density, const uint8_t *labels,
     const int M = ctx->scene.n_materials;
     const float *d_den = density;
     const uint8_t *d_lab = labels;
-    float *tmp_den = nullptr;
-    uint8_t *tmp_lab = nullptr;
-    unsigned int *d_max = nullptr;
+    DevBuf tmp_den, tmp_lab, d_max;
     if (!on_device) {
         for (int64_t i = 0; i < n; i++)
             if (labels[i] >= M)
                 return drr_set_error(DRR_ERR_INVALID, "drr_volume_create: label >= n_materials at voxel " +
                                                           std::to_string(i));
-        DRR_HIP_CHECK(hipMalloc(&tmp_den, sizeof(float) * n));
-        DRR_HIP_CHECK(hipMalloc(&tmp_lab, n));
-        DRR_HIP_CHECK(hipMemcpy(tmp_den, density, sizeof(float) * n, hipMemcpyHostToDevice));
-        DRR_HIP_CHECK(hipMemcpy(tmp_lab, labels, n, hipMemcpyHostToDevice));
-        d_den = tmp_den;
-        d_lab = tmp_lab;
+        DRR_HIP_CHECK(tmp_den.alloc(sizeof(float) * n));
+        DRR_HIP_CHECK(tmp_lab.alloc(n));
+        DRR_HIP_CHECK(hipMemcpy(tmp_den.p, density, sizeof(float) * n, hipMemcpyHostToDevice));
+        DRR_HIP_CHECK(hipMemcpy(tmp_lab.p, labels, n, hipMemcpyHostToDevice));
+        d_den = tmp_den.as<float>();
+        d_lab = tmp_lab.as<uint8_t>();
     } else {
-        DRR_HIP_CHECK(hipMalloc(&d_max, sizeof(unsigned int)));
-        DRR_HIP_CHECK(hipMemset(d_max, 0, sizeof(unsigned int)));
-        k_label_max<<<1024, 256, 0, 0>>>(labels, n, d_max);
+        // device inputs may come from any stream of the caller: wait for all
+        // queued work before reading them
+        DRR_HIP_CHECK(hipDeviceSynchronize());
+        DRR_HIP_CHECK(d_max.alloc(sizeof(unsigned int)));
+        DRR_HIP_CHECK(hipMemset(d_max.p, 0, sizeof(unsigned int)));
+        k_label_max<<<1024, 256, 0, 0>>>(labels, n, d_max.as<unsigned int>());
         DRR_HIP_CHECK(hipGetLastError());
         unsigned int mx = 0;
-        DRR_HIP_CHECK(hipMemcpy(&mx, d_max, sizeof(unsigned int), hipMemcpyDeviceToHost));
-        (void)hipFree(d_max);
+        DRR_HIP_CHECK(hipMemcpy(&mx, d_max.p, sizeof(unsigned int), hipMemcpyDeviceToHost));
         if ((int)mx >= M)
             return drr_set_error(DRR_ERR_INVALID, "drr_volume_create: label >= n_materials");
     }
@@ -243,8 +261,6 @@ int drr_volume_create(drr_ctx *ctx, const float *density, const uint8_t *labels,
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = drr_hip_error(e, "drr_volume_create: expand");
     }
-    (void)hipFree(tmp_den);
-    (void)hipFree(tmp_lab);
     if (rc != DRR_OK) {
         if (v) drr_volume_destroy(v);
         return rc;
@@ -270,35 +286,31 @@ int drr_volume_create_from_hu(drr_ctx *ctx, const int16_t *hu, int nx, int ny, i
             return drr_set_error(DRR_ERR_INVALID, "drr_volume_create_from_hu: label_of_class >= n_materials");
     DeviceGuard g(ctx->device);
     const int64_t n = (int64_t)nx * ny * nz;
-    int16_t *tmp_hu = nullptr;
-    float *d_den = nullptr, *d_upper = nullptr;
-    uint8_t *d_lab = nullptr, *d_map = nullptr;
+    DevBuf tmp_hu, d_den, d_lab, d_upper, d_map;
     const int16_t *src = hu;
     if (!on_device) {
-        DRR_HIP_CHECK(hipMalloc(&tmp_hu, sizeof(int16_t) * n));
-        DRR_HIP_CHECK(hipMemcpy(tmp_hu, hu, sizeof(int16_t) * n, hipMemcpyHostToDevice));
-        src = tmp_hu;
+        DRR_HIP_CHECK(tmp_hu.alloc(sizeof(int16_t) * n));
+        DRR_HIP_CHECK(hipMemcpy(tmp_hu.p, hu, sizeof(int16_t) * n, hipMemcpyHostToDevice));
+        src = tmp_hu.as<int16_t>();
+    } else {
+        DRR_HIP_CHECK(hipDeviceSynchronize());  // inputs from any stream of the caller
     }
-    DRR_HIP_CHECK(hipMalloc(&d_den, sizeof(float) * n));
-    DRR_HIP_CHECK(hipMalloc(&d_lab, n));
-    DRR_HIP_CHECK(hipMalloc(&d_upper, sizeof(float) * n_classes));
-    DRR_HIP_CHECK(hipMalloc(&d_map, n_classes));
-    DRR_HIP_CHECK(hipMemcpy(d_upper, upper_hu, sizeof(float) * n_classes, hipMemcpyHostToDevice));
-    DRR_HIP_CHECK(hipMemcpy(d_map, label_of_class, n_classes, hipMemcpyHostToDevice));
-    k_hu_ingest<<<2048, 256, 0, 0>>>(src, n, n_classes, d_upper, d_map, d_den, d_lab);
+    DRR_HIP_CHECK(d_den.alloc(sizeof(float) * n));
+    DRR_HIP_CHECK(d_lab.alloc(n));
+    DRR_HIP_CHECK(d_upper.alloc(sizeof(float) * n_classes));
+    DRR_HIP_CHECK(d_map.alloc(n_classes));
+    DRR_HIP_CHECK(hipMemcpy(d_upper.p, upper_hu, sizeof(float) * n_classes, hipMemcpyHostToDevice));
+    DRR_HIP_CHECK(hipMemcpy(d_map.p, label_of_class, n_classes, hipMemcpyHostToDevice));
+    k_hu_ingest<<<2048, 256, 0, 0>>>(src, n, n_classes, d_upper.as<float>(), d_map.as<uint8_t>(),
+                                     d_den.as<float>(), d_lab.as<uint8_t>());
     DRR_HIP_CHECK(hipGetLastError());
     drr_volume *v = nullptr;
     rc = volume_alloc(ctx, nx, ny, nz, &v);
-    if (rc == DRR_OK) rc = drr_launch_expand(d_den, d_lab, nx, ny, nz, v->cells, 0);
+    if (rc == DRR_OK) rc = drr_launch_expand(d_den.as<float>(), d_lab.as<uint8_t>(), nx, ny, nz, v->cells, 0);
     if (rc == DRR_OK) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = drr_hip_error(e, "drr_volume_create_from_hu");
     }
-    (void)hipFree(tmp_hu);
-    (void)hipFree(d_den);
-    (void)hipFree(d_lab);
-    (void)hipFree(d_upper);
-    (void)hipFree(d_map);
     if (rc != DRR_OK) {
         if (v) drr_volume_destroy(v);
         return rc;
@@ -309,7 +321,7 @@ int drr_volume_create_from_hu(drr_ctx *ctx, const int16_t *hu, int nx, int ny, i
 
 int drr_volume_destroy(drr_volume *vol) {
     if (!vol) return DRR_OK;
-    DeviceGuard g(vol->ctx ? vol->ctx->device : 0);
+    DeviceGuard g(vol->device);
     (void)hipFree(vol->cells);
     (void)hipFree(vol->lin_vox);
     delete vol;
@@ -381,18 +393,15 @@ extern "C" {
 
 int drr_volume_download(drr_volume *vol, float *density, uint8_t *labels) {
     if (!vol || !density || !labels) return drr_set_error(DRR_ERR_INVALID, "drr_volume_download: NULL argument");
-    DeviceGuard g(vol->ctx->device);
+    DeviceGuard g(vol->device);
     const int64_t n = (int64_t)vol->nx * vol->ny * vol->nz;
-    float *d = nullptr;
-    uint8_t *l = nullptr;
-    DRR_HIP_CHECK(hipMalloc(&d, sizeof(float) * n));
-    DRR_HIP_CHECK(hipMalloc(&l, n));
-    k_unexpand<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, d, l);
+    DevBuf d, l;
+    DRR_HIP_CHECK(d.alloc(sizeof(float) * n));
+    DRR_HIP_CHECK(l.alloc(n));
+    k_unexpand<<<2048, 256, 0, 0>>>(vol->cells, vol->nx, vol->ny, vol->nz, d.as<float>(), l.as<uint8_t>());
     DRR_HIP_CHECK(hipGetLastError());
-    DRR_HIP_CHECK(hipMemcpy(density, d, sizeof(float) * n, hipMemcpyDeviceToHost));
-    DRR_HIP_CHECK(hipMemcpy(labels, l, n, hipMemcpyDeviceToHost));
-    (void)hipFree(d);
-    (void)hipFree(l);
+    DRR_HIP_CHECK(hipMemcpy(density, d.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    DRR_HIP_CHECK(hipMemcpy(labels, l.p, n, hipMemcpyDeviceToHost));
     return DRR_OK;
 }
 

@@ -147,7 +147,7 @@ class MonteCarlo:
 
 def scatter_for_projector(projector, projections: Sequence[geo.CameraProjection]) -> np.ndarray:
     """Scatter intensity to add to ``Projector`` images ([N, H, W], keV per photon
-    aimed at the pixel), from ``projector.scatter_num`` histories per view.
+    aimed at the pixel), from ``projector.mc_scatter_histories`` histories per view.
 
     The MC image is the scattered energy reaching each pixel per emitted photon;
     dividing by the pixel's share of the emission cone (Omega_pixel / Omega_cone)
@@ -156,7 +156,7 @@ def scatter_for_projector(projector, projections: Sequence[geo.CameraProjection]
     if len(projector.volumes) != 1:
         raise NotImplementedError("scatter simulation supports single-volume projectors")
     vol = projector.volumes[0]
-    n_hist = int(projector.scatter_num)
+    n_hist = int(projector.mc_scatter_histories)
     out = []
     for proj in projections:
         p = make_params(projector.all_materials, projector.spectrum_arr, proj,

@@ -108,7 +108,15 @@ class Projector:
         max_mesh_hits: int = 32,
         mesh_layers: int = 2,
         cuda_device_id: Optional[int] = None,
+        mc_scatter_histories: int = 0,
     ) -> None:
+        """Arguments as deepdrr's ``Projector`` (projector.py:398-422), plus
+        ``mc_scatter_histories``: photon histories per view of this engine's
+        Monte-Carlo scatter (Woodcock tracking with the synthetic power-law
+        cross sections of ``mc.SYNTHETIC_MU60``), added to the intensity
+        before noise and neglog; 0 (default) renders no scatter.  As in the
+        reference, ``scatter_num > 0`` (or ``add_scatter=True``) raises
+        ``DeprecationError``."""
         self.cuda_device_id = cuda_device_id
         self.mesh_layers = mesh_layers
         volume = utils.listify(volume)
@@ -152,6 +160,13 @@ class Projector:
             self.scatter_num = scatter_num
         if self.scatter_num > 0 and self.device is None:
             raise ValueError("Must provide device to simulate scatter.")
+        if self.scatter_num > 0:
+            raise DeprecationError("Scatter is deprecated.")
+        if mc_scatter_histories < 0:
+            raise ValueError("mc_scatter_histories must be non-negative.")
+        self.mc_scatter_histories = int(mc_scatter_histories)
+        if self.mc_scatter_histories > 0 and self.device is None:
+            raise ValueError("Must provide device to simulate scatter.")
 
         self.add_noise = add_noise
         self.photon_count = photon_count
@@ -186,6 +201,13 @@ class Projector:
                 Material.from_string(mat)
             except AttributeError:
                 raise ValueError(f"Material {mat} not found in material database. Please check the material name.")
+        if self.mc_scatter_histories > 0:
+            from .mc import SYNTHETIC_MU60
+
+            missing = [m for m in self.all_materials if m not in SYNTHETIC_MU60]
+            if missing:
+                raise ValueError(f"no Monte-Carlo cross sections for materials {missing}; "
+                                 f"known: {sorted(SYNTHETIC_MU60)}")
         self.output_shape = None
         self.initialized = False
         self._ctx = None
@@ -349,7 +371,7 @@ class Projector:
             photon_probs.append(p)
         images = np.stack(intensities)
         photon_prob = np.stack(photon_probs)
-        if self.scatter_num > 0:
+        if self.mc_scatter_histories > 0:
             from .mc import scatter_for_projector
 
             images = images + scatter_for_projector(self, camera_projections)
@@ -381,10 +403,10 @@ class Projector:
         sizes = {tuple(p.intrinsic.sensor_size) for p in camera_projections}
         # noise and neglog run on the device (one D2H copy) when every image has
         # one size and nothing needs the host (scatter, intensity clip)
-        device_noise = (self.add_noise and len(sizes) == 1 and self.scatter_num == 0
+        device_noise = (self.add_noise and len(sizes) == 1 and self.mc_scatter_histories == 0
                         and self.intensity_upper_bound is None)
         host_epilogue = (self.add_noise and not device_noise) or self.intensity_upper_bound is not None \
-            or self.scatter_num > 0
+            or self.mc_scatter_histories > 0
         plan = plan_for_projector(self, camera_projections, volume_indices,
                                   neglog=self.neglog and not host_epilogue and not device_noise, tuning=tuning)
         try:
@@ -397,7 +419,7 @@ class Projector:
                 out = []
                 for k, (img, pp) in enumerate(zip(images, photon_prob)):
                     x = img[None]
-                    if self.scatter_num > 0:
+                    if self.mc_scatter_histories > 0:
                         from .mc import scatter_for_projector
 
                         x = x + scatter_for_projector(self, [camera_projections[k]])

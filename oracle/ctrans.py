@@ -92,6 +92,15 @@ def _mc_lib():
         l.ora_sample_compton.argtypes = [C.c_float, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p]
         l.ora_sample_rayleigh.restype = None
         l.ora_sample_rayleigh.argtypes = [C.c_int, C.c_uint64, C.c_void_p]
+        l.ora_mc_tally.restype = C.c_long
+        l.ora_mc_tally.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(MCParams),
+                                   C.c_int64, C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p]
+        l.ora_rotate.restype = None
+        l.ora_rotate.argtypes = [C.c_float] * 5 + [C.c_void_p]
+        l.ora_ranecu.restype = None
+        l.ora_ranecu.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        l.ora_ranecu_seed.restype = None
+        l.ora_ranecu_seed.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         l._mc_bound = True
     return l
 
@@ -121,3 +130,35 @@ def sample_rayleigh(n, seed=1):
     c = np.zeros(n, np.float32)
     _mc_lib().ora_sample_rayleigh(n, C.c_uint64(seed), c.ctypes.data)
     return c
+
+
+TALLY_KEYS = ("emitted", "scored", "photoelectric", "compton_recoil", "cutoff", "escaped")
+
+
+def mc_tally(density_kji, labels_kji, params, n_histories, seed=12345, first=0):
+    """Energy balance of CPU histories: dict of keV totals (TALLY_KEYS), the
+    scored histories per class and the Woodcock step count."""
+    d = np.ascontiguousarray(density_kji, np.float32)
+    l = np.ascontiguousarray(labels_kji, np.uint8)
+    nz, ny, nx = d.shape
+    tally = np.zeros(len(TALLY_KEYS), np.float64)
+    counts = np.zeros(4, np.int64)
+    steps = _mc_lib().ora_mc_tally(d.ctypes.data, l.ctypes.data, nx, ny, nz, C.byref(params), first,
+                                   n_histories, C.c_uint64(seed), tally.ctypes.data, counts.ctypes.data)
+    return dict(zip(TALLY_KEYS, tally.tolist())), counts, int(steps)
+
+
+def rotate(u, v, w, ct, phi):
+    out = np.zeros(3, np.float32)
+    _mc_lib().ora_rotate(float(u), float(v), float(w), float(ct), float(phi), out.ctypes.data)
+    return out
+
+
+def ranecu(seed, h, n):
+    s1 = np.zeros(n, np.int32)
+    s2 = np.zeros(n, np.int32)
+    u = np.zeros(n, np.float32)
+    _mc_lib().ora_ranecu(C.c_uint64(seed), C.c_uint64(h), n, s1.ctypes.data, s2.ctypes.data, u.ctypes.data)
+    a, b = C.c_int(), C.c_int()
+    _mc_lib().ora_ranecu_seed(C.c_uint64(seed), C.c_uint64(h), C.byref(a), C.byref(b))
+    return (a.value, b.value), s1, s2, u

@@ -125,10 +125,16 @@ static float dcoord(const float *A, int a, const float *d) {
     return A[4 * a] * d[0] + A[4 * a + 1] * d[1] + A[4 * a + 2] * d[2];
 }
 
+/* Energy bookkeeping of one history, in keV (ora_mc_tally): where the emitted
+ * energy ends.  Indices of tally[]. */
+enum { T_EMITTED, T_SCORED, T_PHOTO, T_COMPTON, T_CUTOFF, T_ESCAPED, T_N };
+
 /* One history: returns class (0..3) and energy via *E_out, or -1 if not scored;
- * *steps counts Woodcock flights. */
+ * *steps counts Woodcock flights; tally (may be NULL) accumulates the energy
+ * balance of the history. */
 static int history(const float *dens, const uint8_t *lab, const int *n, const drr_mc_params_t *p,
-                   uint64_t seed, uint64_t h, float *E_out, int *iu_out, int *iv_out, long *steps) {
+                   uint64_t seed, uint64_t h, float *E_out, int *iu_out, int *iv_out, long *steps,
+                   double *tally) {
     ranecu_t rng = seed_history(seed, h);
     float u = ranecu(&rng);
     int lo = 0, hi = p->n_bins - 1;
@@ -137,6 +143,7 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
         if (p->cdf[mid] >= u) hi = mid; else lo = mid + 1;
     }
     float E = p->energies_keV[lo];
+    if (tally) tally[T_EMITTED] += E;
     float d[3] = {p->cone_axis[0], p->cone_axis[1], p->cone_axis[2]};
     float ct = 1.0f - ranecu(&rng) * (1.0f - p->cone_cos);
     rotate(&d[0], &d[1], &d[2], ct, TWO_PI * ranecu(&rng));
@@ -183,16 +190,24 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
             float mt = mpe + mc + mr;
             if (ranecu(&rng) * mumax < mt) {
                 float uu = ranecu(&rng) * mt;
-                if (uu < mpe) return -1;
+                if (uu < mpe) {
+                    if (tally) tally[T_PHOTO] += E;
+                    return -1;
+                }
                 if (uu < mpe + mc) {
+                    const float E0 = E;
                     ct = compton(&rng, &E);
+                    if (tally) tally[T_COMPTON] += (double)E0 - (double)E;
                     nc++;
                 } else {
                     ct = rayleigh(&rng);
                     nr++;
                 }
                 rotate(&d[0], &d[1], &d[2], ct, TWO_PI * ranecu(&rng));
-                if (E < E_MIN || nc + nr > p->max_interactions) return -1;
+                if (E < E_MIN || nc + nr > p->max_interactions) {
+                    if (tally) tally[T_CUTOFF] += E;
+                    return -1;
+                }
                 float tm = INFINITY;
                 for (int a = 0; a < 3; a++) {
                     float q0 = qcoord(A, a, x), dq = dcoord(A, a, d);
@@ -205,16 +220,26 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
     }
     const float *N = p->detector_normal, *O = p->detector_origin, *U = p->detector_u, *V = p->detector_v;
     float dn = d[0] * N[0] + d[1] * N[1] + d[2] * N[2];
-    if (dn <= 0.0f) return -1;
+    if (dn <= 0.0f) {
+        if (tally) tally[T_ESCAPED] += E;
+        return -1;
+    }
     float t = ((O[0] - x[0]) * N[0] + (O[1] - x[1]) * N[1] + (O[2] - x[2]) * N[2]) / dn;
-    if (t < 0.0f) return -1;
+    if (t < 0.0f) {
+        if (tally) tally[T_ESCAPED] += E;
+        return -1;
+    }
     float r[3];
     for (int a = 0; a < 3; a++) r[a] = x[a] + t * d[a] - O[a];
     float uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
     float vv = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
     float fu = (r[0] * U[0] + r[1] * U[1] + r[2] * U[2]) / uu;
     float fv = (r[0] * V[0] + r[1] * V[1] + r[2] * V[2]) / vv;
-    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)p->det_width && fv < (float)p->det_height)) return -1;
+    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)p->det_width && fv < (float)p->det_height)) {
+        if (tally) tally[T_ESCAPED] += E;
+        return -1;
+    }
+    if (tally) tally[T_SCORED] += E;
     int iu = (int)fu, iv = (int)fv;
     if (iu > p->det_width - 1) iu = p->det_width - 1;
     if (iv > p->det_height - 1) iv = p->det_height - 1;
@@ -240,7 +265,7 @@ long ora_mc(const float *dens, const uint8_t *lab, int nx, int ny, int nz, const
         for (int64_t i = 0; i < count; i++) {
             float E;
             int iu, iv;
-            int cls = history(dens, lab, n, p, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps);
+            int cls = history(dens, lab, n, p, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, NULL);
             if (cls >= 0) {
                 long o = cls * npix + (long)iv * p->det_width + iu;
                 li[o] += E;
@@ -275,4 +300,51 @@ void ora_sample_compton(float E, int n, uint64_t seed, float *cos_out, float *e_
 void ora_sample_rayleigh(int n, uint64_t seed, float *cos_out) {
     ranecu_t r = seed_history(seed, 0);
     for (int i = 0; i < n; i++) cos_out[i] = rayleigh(&r);
+}
+
+/* Energy balance of histories [first, first + count): tally[6] receives, in
+ * keV, the emitted energy and where it ended -- scored on the detector,
+ * absorbed by photoelectric events, deposited by Compton recoils, dropped by
+ * the energy / interaction cutoff, escaped past the detector.  counts[4]
+ * receives the scored histories per class.  Single-threaded. */
+long ora_mc_tally(const float *dens, const uint8_t *lab, int nx, int ny, int nz, const drr_mc_params_t *p,
+                  int64_t first, int64_t count, uint64_t seed, double *tally, long *counts) {
+    const int n[3] = {nx, ny, nz};
+    long steps = 0;
+    for (int k = 0; k < T_N; k++) tally[k] = 0.0;
+    for (int k = 0; k < 4; k++) counts[k] = 0;
+    for (int64_t i = 0; i < count; i++) {
+        float E;
+        int iu, iv;
+        int cls = history(dens, lab, n, p, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, tally);
+        if (cls >= 0) counts[cls]++;
+    }
+    return steps;
+}
+
+/* The direction rotation of an interaction: (u, v, w) turned by polar cosine
+ * ct and azimuth phi (out[3]). */
+void ora_rotate(float u, float v, float w, float ct, float phi, float *out) {
+    rotate(&u, &v, &w, ct, phi);
+    out[0] = u;
+    out[1] = v;
+    out[2] = w;
+}
+
+/* The first n draws of history h's RANECU stream: the two generator states
+ * after each draw and the uniform it returned. */
+void ora_ranecu(uint64_t seed, uint64_t h, int n, int *s1, int *s2, float *u) {
+    ranecu_t r = seed_history(seed, h);
+    for (int i = 0; i < n; i++) {
+        u[i] = ranecu(&r);
+        s1[i] = r.s1;
+        s2[i] = r.s2;
+    }
+}
+
+/* The seeds of history h's stream (before any draw). */
+void ora_ranecu_seed(uint64_t seed, uint64_t h, int *s1, int *s2) {
+    ranecu_t r = seed_history(seed, h);
+    *s1 = r.s1;
+    *s2 = r.s2;
 }

@@ -120,6 +120,18 @@ def test_projector_constructor_errors():
         Projector(vol, scatter_num=-1)
     with pytest.raises(ValueError):
         Projector(vol, scatter_num=10)  # needs a device
+    # as in the reference (projector.py:530-531), scatter_num > 0 is refused
+    from deepdrr_amd.projector import DeprecationError
+
+    dev = phantoms.ap_device(8, 8, 1.0)
+    with pytest.raises(DeprecationError):
+        Projector(vol, device=dev, scatter_num=10)
+    with pytest.raises(DeprecationError):
+        Projector(vol, device=dev, add_scatter=True)
+    with pytest.raises(ValueError):
+        Projector(vol, mc_scatter_histories=10)  # needs a device
+    with pytest.raises(ValueError):
+        Projector(vol, mc_scatter_histories=-1)
     with pytest.raises(ValueError):
         Projector(vol, max_mesh_hits=6)
     with pytest.raises(KeyError):

@@ -77,7 +77,7 @@ def test_mc_matches_cpu(gpu):
 
 
 def test_projector_with_scatter(gpu):
-    """scatter_num > 0 adds the MC scatter estimate to the images (the reference raises)."""
+    """mc_scatter_histories > 0 adds the MC scatter estimate to the images."""
     from deepdrr_amd import Projector
     from deepdrr_amd.spectral_data import synthetic_120kv
 
@@ -85,7 +85,7 @@ def test_projector_with_scatter(gpu):
     dev = phantoms.ap_device(16, 16, 4.0, sdd=600.0)
     with Projector(vol, device=dev, spectrum=synthetic_120kv(), neglog=False) as p0:
         base = p0()
-    with Projector(vol, device=dev, spectrum=synthetic_120kv(), neglog=False, scatter_num=200000) as p1:
+    with Projector(vol, device=dev, spectrum=synthetic_120kv(), neglog=False, mc_scatter_histories=200000) as p1:
         withs = p1()
     assert withs.shape == base.shape
     assert np.all(withs >= base - 1e-3) and np.mean(withs - base) > 0
