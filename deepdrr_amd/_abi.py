@@ -39,6 +39,7 @@ SIGNATURES = {
     "drr_batch_last_march_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "drr_batch_destroy": (C.c_int, [_P]),
     "drr_project_jobs": (C.c_int, [_P, _P, C.c_int, C.POINTER(Job), C.c_int, _P, _P, _P, _P]),
+    "drr_project_jobs_noalloc": (C.c_int, [_P, _P, C.c_int, C.POINTER(Job), C.c_int, _P, _P, _P, _P]),
     "drr_adjoint_jobs": (C.c_int, [_P, _P, C.POINTER(Job), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
     "drr_neglog": (C.c_int, [_P, _P, _P, C.c_int, C.c_int64, C.c_float, _P]),
     "drr_add_noise": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),

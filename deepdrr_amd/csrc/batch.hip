@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "drr_internal.h"
@@ -44,51 +46,85 @@ int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, cons
     return drr_jobs_build_tiles(ctx, vols, n_vols, jobs, n_jobs, tune, t);
 }
 
-int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
-                         int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
+namespace {
+// Host half of a job table: validated jobs, volume descriptors, the tile list
+// grouped by volume and job, and the 8 queue segments.
+struct JobHost {
+    std::vector<JobDev> jobs;
+    std::vector<VolDev> vols;
+    std::vector<int2> items;
+    int seg[9];
+};
+
+// Validate the jobs and fill `h` and the host fields of `t` (n_jobs, n_vols,
+// imgs, max_npix, n_items, grid, block).  A plan with more than `max_items`
+// ray tiles is refused before its tile list is built.
+int jobs_prepare(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs, int n_jobs,
+                 const drr_batch_tuning_t &tune, int64_t max_items, const char *who, JobTable &t, JobHost &h) {
     if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
-        return drr_set_error(DRR_ERR_INVALID, "jobs: need >= 1 volume and 1..65535 jobs");
+        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": need >= 1 volume and 1..65535 jobs");
     if (tune.tile_w < 1 || tune.tile_h < 1 || tune.waves_per_block < 1 || tune.blocks_per_cu < 1)
-        return drr_set_error(DRR_ERR_INVALID, "jobs: bad tiling");
+        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": bad tiling");
     for (int v = 0; v < n_vols; v++)
-        if (!vols[v] || vols[v]->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "jobs: NULL or foreign volume");
+        if (!vols[v] || vols[v]->ctx != ctx)
+            return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": NULL or foreign volume");
     t.n_jobs = n_jobs;
     t.n_vols = n_vols;
-    std::vector<JobDev> hj(n_jobs);
+    h.jobs.assign(n_jobs, JobDev{});
     t.imgs.assign(n_jobs, ImgSeg{0, 0});
     std::vector<int> order(n_jobs);
     t.max_npix = 0;
+    int64_t n_items = 0;
     for (int j = 0; j < n_jobs; j++) {
         const drr_job_t &J = jobs[j];
         if (J.view.n_volumes != 1)
-            return drr_set_error(DRR_ERR_UNSUPPORTED, "jobs: each job must view exactly one volume");
+            return drr_set_error(DRR_ERR_UNSUPPORTED, std::string(who) + ": each job must view exactly one volume");
         if (J.volume_index < 0 || J.volume_index >= n_vols)
-            return drr_set_error(DRR_ERR_INVALID, "jobs: volume_index out of range");
+            return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": volume_index out of range");
         if (J.view.width < 1 || J.view.height < 1 || J.view.width > 65536 || J.view.height > 65536 || J.out_offset < 0)
-            return drr_set_error(DRR_ERR_INVALID, "jobs: bad image size or offset");
+            return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": bad image size or offset");
         if (drr_check_view_finite(&J.view) != DRR_OK) return DRR_ERR_INVALID;
-        view_to_geo1(J.view, hj[j].g);
-        hj[j].vol = J.volume_index;
-        hj[j].tiles_x = (J.view.width + tune.tile_w - 1) / tune.tile_w;
-        hj[j].out_offset = J.out_offset;
+        view_to_geo1(J.view, h.jobs[j].g);
+        h.jobs[j].vol = J.volume_index;
+        h.jobs[j].tiles_x = (J.view.width + tune.tile_w - 1) / tune.tile_w;
+        h.jobs[j].out_offset = J.out_offset;
         t.imgs[j] = ImgSeg{J.out_offset, (int64_t)J.view.width * J.view.height};
         t.max_npix = std::max<int64_t>(t.max_npix, t.imgs[j].npix);
+        n_items += (int64_t)h.jobs[j].tiles_x * ((J.view.height + tune.tile_h - 1) / tune.tile_h);
         order[j] = j;
     }
+    if (n_items >= ((int64_t)1 << 31)) return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": too many ray tiles");
+    if (max_items >= 0 && n_items > max_items)
+        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": the batch has " + std::to_string(n_items) +
+                                                  " ray tiles, more than the " + std::to_string(max_items) +
+                                                  " its scratch holds");
     // group tiles by volume (L2 / MALL reuse across poses), then by job
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hj[a].vol < hj[b].vol; });
-    std::vector<int2> items;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h.jobs[a].vol < h.jobs[b].vol; });
+    h.items.clear();
+    h.items.reserve((size_t)n_items);
     for (int j : order) {
         const int ty = (jobs[j].view.height + tune.tile_h - 1) / tune.tile_h;
-        const int n = hj[j].tiles_x * ty;
-        for (int k = 0; k < n; k++) items.push_back(make_int2(j, k));
+        const int n = h.jobs[j].tiles_x * ty;
+        for (int k = 0; k < n; k++) h.items.push_back(make_int2(j, k));
     }
-    t.n_items = (int64_t)items.size();
-    if (t.n_items >= ((int64_t)1 << 31)) return drr_set_error(DRR_ERR_INVALID, "jobs: too many ray tiles");
-    int seg[9];
-    for (int q = 0; q <= 8; q++) seg[q] = (int)(t.n_items * q / 8);
-    std::vector<VolDev> hv(n_vols);
-    for (int v = 0; v < n_vols; v++) hv[v] = vol_to_dev(vols[v]);
+    t.n_items = n_items;
+    for (int q = 0; q <= 8; q++) h.seg[q] = (int)(t.n_items * q / 8);
+    h.vols.resize(n_vols);
+    for (int v = 0; v < n_vols; v++) h.vols[v] = vol_to_dev(vols[v]);
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e != hipSuccess) return drr_hip_error(e, "hipDeviceGetAttribute(multiprocessor count)");
+    t.block = 64 * tune.waves_per_block;
+    t.grid = cus * tune.blocks_per_cu;
+    return DRR_OK;
+}
+}  // namespace
+
+int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                         int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
+    JobHost h;
+    int rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, -1, "jobs", t, h);
+    if (rc != DRR_OK) return rc;
     hipError_t e;
 #define JCHK(x)                                             \
     do {                                                    \
@@ -100,18 +136,14 @@ int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols
     } while (0)
     JCHK(hipMalloc(&t.d_jobs, sizeof(JobDev) * n_jobs));
     JCHK(hipMalloc(&t.d_vols, sizeof(VolDev) * n_vols));
-    JCHK(hipMalloc(&t.d_items, sizeof(int2) * items.size()));
+    JCHK(hipMalloc(&t.d_items, sizeof(int2) * h.items.size()));
     JCHK(hipMalloc(&t.d_seg, sizeof(int) * 9));
     JCHK(hipMalloc(&t.d_counters, sizeof(int) * 8));
-    JCHK(hipMemcpy(t.d_jobs, hj.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
-    JCHK(hipMemcpy(t.d_vols, hv.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice));
-    JCHK(hipMemcpy(t.d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
-    JCHK(hipMemcpy(t.d_seg, seg, sizeof(int) * 9, hipMemcpyHostToDevice));
-    int cus = 0;
-    JCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    JCHK(hipMemcpy(t.d_jobs, h.jobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_vols, h.vols.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice));
+    JCHK(hipMemcpy(t.d_seg, h.seg, sizeof(int) * 9, hipMemcpyHostToDevice));
 #undef JCHK
-    t.block = 64 * tune.waves_per_block;
-    t.grid = cus * tune.blocks_per_cu;
     return DRR_OK;
 }
 
@@ -330,5 +362,103 @@ extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int
     drr_jobs_free(t);
     if (e != hipSuccess) return drr_hip_error(e, "drr_project_jobs launch");
     if (e2 != hipSuccess) return drr_hip_error(e2, "drr_project_jobs");
+    return DRR_OK;
+}
+
+// ---- drr_project_jobs without device allocation ------------------------------
+//
+// The job table, volume descriptors, queue segments and counters and the tile
+// list of one call live in `scratch`, a file-scope device array of
+// DRR_NOALLOC_SCRATCH_BYTES that every device holds until the process ends.
+// In host code `scratch` names a host-side stand-in, so its device address is
+// looked up once per device (hipGetSymbolAddress, with that device current)
+// and only that address is handed to copies and kernels.  All calls on one
+// device share the array: calls that overlap in time on one device race.
+
+static __device__ __align__(256) unsigned char scratch[DRR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device).
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices)
+        return drr_set_error(DRR_ERR_UNSUPPORTED, "drr_project_jobs_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return drr_hip_error(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return drr_set_error(DRR_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return DRR_OK;
+}
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *vols, int n_vols,
+                                        const drr_job_t *jobs, int n_jobs, float *intensity,
+                                        float *photon_prob, float *area_density, void *stream) {
+    if (!ctx || !vols || !jobs || !intensity)
+        return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs_noalloc: NULL argument");
+    if (!ctx->have_scene || ctx->n_bins < 1)
+        return drr_set_error(DRR_ERR_STATE, "drr_project_jobs_noalloc: scene/spectrum not set");
+    if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
+        return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs_noalloc: need >= 1 volume and 1..65535 jobs");
+    DevSwitch ds(ctx->device);
+    unsigned char *base = nullptr;
+    int rc = scratch_address(ctx->device, &base);
+    if (rc != DRR_OK) return rc;
+    // layout: counters | segments | volumes | jobs | tiles, each 256-B aligned
+    const size_t off_seg = 256, off_vols = 512;
+    const size_t off_jobs = align256(off_vols + sizeof(VolDev) * (size_t)n_vols);
+    const size_t off_items = align256(off_jobs + sizeof(JobDev) * (size_t)n_jobs);
+    if (off_items > DRR_NOALLOC_SCRATCH_BYTES)
+        return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs_noalloc: the job table exceeds the " +
+                                                  std::to_string(DRR_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+    const int64_t max_items = (int64_t)((DRR_NOALLOC_SCRATCH_BYTES - off_items) / sizeof(int2));
+    JobTable t;
+    JobHost h;
+    const drr_batch_tuning_t tune{8, 8, 4, 8};
+    rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, max_items, "drr_project_jobs_noalloc", t, h);
+    if (rc != DRR_OK) return rc;
+    t.d_counters = (int *)base;
+    t.d_seg = (int *)(base + off_seg);
+    t.d_vols = (VolDev *)(base + off_vols);
+    t.d_jobs = (JobDev *)(base + off_jobs);
+    t.d_items = (int2 *)(base + off_items);
+    hipStream_t st = (hipStream_t)stream;
+    // The host tables must outlive the copies that read them: every return
+    // after the first copy is queued synchronizes the stream first.
+    hipError_t e;
+#define NCHK(x)                                      \
+    do {                                             \
+        e = (x);                                     \
+        if (e != hipSuccess) {                       \
+            (void)hipStreamSynchronize(st);          \
+            return drr_hip_error(e, #x);             \
+        }                                            \
+    } while (0)
+    NCHK(hipMemcpyAsync(t.d_seg, h.seg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
+    NCHK(hipMemcpyAsync(t.d_vols, h.vols.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice, st));
+    NCHK(hipMemcpyAsync(t.d_jobs, h.jobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, st));
+    NCHK(hipMemcpyAsync(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice, st));
+#undef NCHK
+    SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
+    (void)hipEventRecord(ctx->ev0, st);
+    k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);  // the previous call's counters are left in scratch
+    launch_march(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
+                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
+    e = hipGetLastError();
+    (void)hipEventRecord(ctx->ev1, st);
+    ctx->timed = true;
+    hipError_t e2 = hipStreamSynchronize(st);
+    if (e != hipSuccess) return drr_hip_error(e, "drr_project_jobs_noalloc launch");
+    if (e2 != hipSuccess) return drr_hip_error(e2, "drr_project_jobs_noalloc");
     return DRR_OK;
 }

@@ -170,6 +170,23 @@ int drr_batch_destroy(drr_batch *batch);
 int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                      int n_jobs, float *intensity, float *photon_prob, float *area_density, void *stream);
 
+/* drr_project_jobs for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, checks and
+ * results as drr_project_jobs, but the job table, queue state and ray-tile
+ * list live in a fixed device array of DRR_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every device it runs on until the process ends; nothing
+ * is allocated or freed on the device.  A batch whose tables exceed that
+ * array (8 bytes per 64-pixel tile plus 152 bytes per job: about 1e6 tiles,
+ * or 28 views of 1536^2) is refused with DRR_ERR_INVALID.  All calls on one
+ * device share the array, so calls that overlap in time on one device race;
+ * each call returns after its launch completes, so calls made one after
+ * another (from one thread) never overlap. */
+#define DRR_NOALLOC_SCRATCH_BYTES (8u << 20)
+int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                             int n_jobs, float *intensity, float *photon_prob, float *area_density,
+                             void *stream);
+
 /* ---- device epilogue --------------------------------------------------------- */
 
 /* Per-image neglog of n images of `npix` pixels at consecutive offsets:
