@@ -35,6 +35,10 @@ def lib():
         l.ora_project.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(View), C.POINTER(Scene),
                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_int]
+        l.ora_project_rows.restype = C.c_int
+        l.ora_project_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(View), C.POINTER(Scene),
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         l.ora_solid_angle.restype = None
         l.ora_solid_angle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         _lib = l
@@ -48,9 +52,11 @@ def default_threads() -> int:
         return os.cpu_count() or 1
 
 
-def project(volumes, view: View, scene: Scene, energies, pdf, mu, n_threads: int | None = None):
+def project(volumes, view: View, scene: Scene, energies, pdf, mu, n_threads: int | None = None, rows=None):
     """volumes: list of (density [nz,ny,nx] f32, labels [nz,ny,nx] u8).
-    Returns intensity [H,W], photon_prob [H,W], area_density [M,H,W] (float32)."""
+    Returns intensity [H,W], photon_prob [H,W], area_density [M,H,W] (float32);
+    with ``rows`` (detector row indices) only those rows, as [len(rows), W]
+    images, each ray computed exactly as in the full view."""
     n_threads = n_threads or default_threads()
     V = view.n_volumes
     dens = [np.ascontiguousarray(d, dtype=np.float32) for d, _ in volumes[:V]]
@@ -58,16 +64,23 @@ def project(volumes, view: View, scene: Scene, energies, pdf, mu, n_threads: int
     dptr = (C.c_void_p * V)(*[d.ctypes.data for d in dens])
     lptr = (C.c_void_p * V)(*[l.ctypes.data for l in labs])
     dims = np.array([s for d in dens for s in d.shape[::-1]], dtype=np.int32)
-    H, W, M = view.height, view.width, scene.n_materials
+    W, M = view.width, scene.n_materials
+    rsel = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    H = view.height if rsel is None else len(rsel)
     I = np.zeros((H, W), np.float32)
     P = np.zeros((H, W), np.float32)
     AD = np.zeros((M, H, W), np.float32)
     e = np.ascontiguousarray(energies, np.float32)
     p = np.ascontiguousarray(pdf, np.float32)
     m = np.ascontiguousarray(mu, np.float32)
-    rc = lib().ora_project(dptr, lptr, dims.ctypes.data, C.byref(view), C.byref(scene), len(e),
-                           e.ctypes.data, p.ctypes.data, m.ctypes.data, I.ctypes.data, P.ctypes.data,
-                           AD.ctypes.data, int(n_threads))
+    if rsel is None:
+        rc = lib().ora_project(dptr, lptr, dims.ctypes.data, C.byref(view), C.byref(scene), len(e),
+                               e.ctypes.data, p.ctypes.data, m.ctypes.data, I.ctypes.data, P.ctypes.data,
+                               AD.ctypes.data, int(n_threads))
+    else:
+        rc = lib().ora_project_rows(dptr, lptr, dims.ctypes.data, C.byref(view), C.byref(scene), len(e),
+                                    e.ctypes.data, p.ctypes.data, m.ctypes.data, rsel.ctypes.data, len(rsel),
+                                    I.ctypes.data, P.ctypes.data, AD.ctypes.data, int(n_threads))
     if rc != 0:
         raise ValueError("ora_project rejected its arguments")
     return I, P, AD

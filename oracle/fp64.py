@@ -25,10 +25,15 @@ F32 = np.float32
 AIR_DENSITY = 0.1129
 
 
-def _ray_setup(view, width, height):
-    """float32 ray directions and lengths, pixel order row-major [v][u]."""
+def _ray_setup(view, width, height, pixels=None):
+    """float32 ray directions and lengths, pixel order row-major [v][u], or of
+    the flat pixel indices v * width + u in ``pixels``."""
     W = np.asarray(view["world_from_index"], dtype=F32).reshape(9)
-    vv, uu = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    if pixels is None:
+        vv, uu = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    else:
+        pix = np.asarray(pixels, np.int64)
+        vv, uu = pix // width, pix % width
     u = (uu.reshape(-1).astype(F32).astype(np.float64) + 0.5).astype(F32)
     v = (vv.reshape(-1).astype(F32).astype(np.float64) + 0.5).astype(F32)
     rx = (u * W[0] + v * W[1]) + W[2]
@@ -39,13 +44,15 @@ def _ray_setup(view, width, height):
     return rx * inv, ry * inv, rz * inv, L
 
 
-def project_fp64(volumes, view, scene, energies, pdf, mu):
+def project_fp64(volumes, view, scene, energies, pdf, mu, pixels=None):
     """volumes: list of (density [nz,ny,nx] f32, labels [nz,ny,nx] u8).
     view: dict(world_from_index[9], max_ray_length, width, height,
                vols=[dict(ijk_from_world[12], source_ijk[3], bounds_lo[3],
                           bounds_hi[3], priority, enabled)])
     scene: dict(step, n_materials, attenuate_outside, air_index)
-    Returns intensity [H,W], photon_prob [H,W], area_density [M,H,W] (float64).
+    Returns intensity [H,W], photon_prob [H,W], area_density [M,H,W] (float64);
+    with ``pixels`` (flat indices v * W + u) only those rays, as intensity [R],
+    photon_prob [R], area_density [M,R].
     """
     Wd, Hd = int(view["width"]), int(view["height"])
     V = len(view["vols"])
@@ -53,7 +60,7 @@ def project_fp64(volumes, view, scene, energies, pdf, mu):
     step = F32(scene["step"])
     atten = bool(scene["attenuate_outside"])
     air = int(scene["air_index"])
-    rx, ry, rz, L = _ray_setup(view, Wd, Hd)
+    rx, ry, rz, L = _ray_setup(view, Wd, Hd, pixels)
     R = rx.shape[0]
     r32 = np.stack([rx, ry, rz], axis=1)
 
@@ -190,6 +197,8 @@ def project_fp64(volumes, view, scene, energies, pdf, mu):
     pp = np.exp(-expo) * P_[:, None]
     photon_prob = pp.sum(0)
     intensity = (E[:, None] * pp).sum(0)
+    if pixels is not None:
+        return intensity, photon_prob, ad
     return (intensity.reshape(Hd, Wd), photon_prob.reshape(Hd, Wd), ad.reshape(M, Hd, Wd))
 
 

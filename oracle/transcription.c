@@ -269,14 +269,18 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
 }
 
 /*
- * Project one view.  density[v], labels[v]: [nz][ny][nx] with dims[3v..3v+2] =
- * (nx, ny, nz).  Outputs [H][W]; area_density (may be NULL) [M][H][W].
- * Rays are split over `n_threads` OpenMP threads (dynamic schedule).
+ * Project detector rows of one view.  density[v], labels[v]: [nz][ny][nx] with
+ * dims[3v..3v+2] = (nx, ny, nz).  rows: NULL for every row (outputs [H][W]),
+ * else n_rows row indices (outputs [n_rows][W], row r of the output is
+ * detector row rows[r]); area_density (may be NULL) is [M][rows][W].  Each
+ * ray is computed exactly as in the full view.  Rays are split over
+ * `n_threads` OpenMP threads (dynamic schedule).
  */
-int ora_project(const float *const *density, const uint8_t *const *labels, const int *dims,
-                const drr_view_t *view, const drr_scene_t *scene, int n_bins,
-                const float *energies, const float *pdf, const float *mu, float *intensity,
-                float *photon_prob, float *area_density, int n_threads) {
+static int ora_project_impl(const float *const *density, const uint8_t *const *labels, const int *dims,
+                            const drr_view_t *view, const drr_scene_t *scene, int n_bins,
+                            const float *energies, const float *pdf, const float *mu, const int *rows,
+                            int n_rows, float *intensity, float *photon_prob, float *area_density,
+                            int n_threads) {
     if (view->n_volumes < 1 || view->n_volumes > DRR_MAX_VOLUMES) return -1;
     if (scene->n_materials < 1 || scene->n_materials > DRR_MAX_MATERIALS) return -1;
     ora_volume vols[DRR_MAX_VOLUMES];
@@ -288,10 +292,15 @@ int ora_project(const float *const *density, const uint8_t *const *labels, const
         vols[v].n[2] = dims[3 * v + 2];
     }
     const int Wd = view->width, Hd = view->height, M = scene->n_materials;
-    const long npix = (long)Wd * Hd;
+    const int Ho = rows ? n_rows : Hd;
+    if (rows)
+        for (int r = 0; r < n_rows; r++)
+            if (rows[r] < 0 || rows[r] >= Hd) return -1;
+    const long npix = (long)Wd * Ho;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads)
     for (long idx = 0; idx < npix; idx++) {
-        int vdx = (int)(idx / Wd), udx = (int)(idx % Wd);
+        int r = (int)(idx / Wd), udx = (int)(idx % Wd);
+        int vdx = rows ? rows[r] : r;
         float ad[DRR_MAX_MATERIALS];
         ora_ray(vols, view, scene, udx, vdx, n_bins, energies, pdf, mu, &intensity[idx],
                 &photon_prob[idx], ad);
@@ -299,6 +308,24 @@ int ora_project(const float *const *density, const uint8_t *const *labels, const
             for (int m = 0; m < M; m++) area_density[(long)m * npix + idx] = ad[m];
     }
     return 0;
+}
+
+int ora_project(const float *const *density, const uint8_t *const *labels, const int *dims,
+                const drr_view_t *view, const drr_scene_t *scene, int n_bins,
+                const float *energies, const float *pdf, const float *mu, float *intensity,
+                float *photon_prob, float *area_density, int n_threads) {
+    return ora_project_impl(density, labels, dims, view, scene, n_bins, energies, pdf, mu, NULL, 0,
+                            intensity, photon_prob, area_density, n_threads);
+}
+
+int ora_project_rows(const float *const *density, const uint8_t *const *labels, const int *dims,
+                     const drr_view_t *view, const drr_scene_t *scene, int n_bins,
+                     const float *energies, const float *pdf, const float *mu, const int *rows,
+                     int n_rows, float *intensity, float *photon_prob, float *area_density,
+                     int n_threads) {
+    if (!rows || n_rows < 1) return -1;
+    return ora_project_impl(density, labels, dims, view, scene, n_bins, energies, pdf, mu, rows, n_rows,
+                            intensity, photon_prob, area_density, n_threads);
 }
 
 /* Solid angle of each pixel seen from the source (project_kernel.cu:14-133),

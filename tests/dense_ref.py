@@ -19,11 +19,17 @@ import torch.nn.functional as F
 F32 = np.float32
 
 
-def _lattice(view, step):
-    """Per ray: unit world direction (f32), alpha_t list and weights of in-range samples."""
+def _ray_setup(view, step, pixels=None):
+    """Per ray: unit world direction (f32), minAlpha, the volume's [mn, mx] and
+    the step count N; rays of every pixel [v][u], or of the flat pixel indices
+    v * W + u in ``pixels`` (each computed as in the full view)."""
     W = np.array(view.world_from_index[:], F32)
     w, h = view.width, view.height
-    vv, uu = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    if pixels is None:
+        vv, uu = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    else:
+        pix = np.asarray(pixels, np.int64)
+        vv, uu = pix // w, pix % w
     u = uu.reshape(-1).astype(F32) + F32(0.5)
     v = vv.reshape(-1).astype(F32) + F32(0.5)
     rx = (u * W[0] + v * W[1]) + W[2]
@@ -51,9 +57,15 @@ def _lattice(view, step):
     minA = np.fmin(L, mn)
     maxA = np.fmax(F32(0), mx)
     N = np.ceil(((maxA - minA) / F32(step)).astype(F32)).astype(np.int64)
+    return r, minA, mn, mx, N
+
+
+def _walk(minA, mn, mx, N, step):
+    """The kernel's sequential FP32 alpha lattice of every ray at once: ray
+    index, alpha and end-sample weight of each in-range sample."""
     rays, alphas, weights = [], [], []
     alpha = minA.copy()
-    for t in range(int(N.max())):
+    for t in range(int(N.max()) if N.size else 0):
         act = t < N
         inr = act & (alpha >= mn) & (alpha <= mx)
         idx = np.nonzero(inr)[0]
@@ -63,7 +75,16 @@ def _lattice(view, step):
             wt = np.where((t == 0) | (N[idx] - 1 == t), 0.5, 1.0)
             weights.append(wt)
         alpha = np.where(act, (alpha + F32(step)).astype(F32), alpha)
-    return (r.astype(np.float64), np.concatenate(rays), np.concatenate(alphas), np.concatenate(weights))
+    if not rays:
+        return np.zeros(0, np.int64), np.zeros(0), np.zeros(0)
+    return np.concatenate(rays), np.concatenate(alphas), np.concatenate(weights)
+
+
+def _lattice(view, step, pixels=None):
+    """Per ray: unit world direction (f32), alpha_t list and weights of in-range samples."""
+    r, minA, mn, mx, N = _ray_setup(view, step, pixels)
+    rays, alphas, wts = _walk(minA, mn, mx, N, step)
+    return r.astype(np.float64), rays, alphas, wts
 
 
 def rodrigues_t(w):
@@ -75,9 +96,11 @@ def rodrigues_t(w):
 
 
 class DenseDRR:
-    """views: list of (ctypes View of the unperturbed pose, source_world (3,))."""
+    """views: list of (ctypes View of the unperturbed pose, source_world (3,)).
+    pixels: None for every pixel (render returns [n_views, H, W]), or flat pixel
+    indices v * W + u shared by all views (render returns [n_views, len(pixels)])."""
 
-    def __init__(self, views, world_from_ijk, center, step, E, pdf, mu, shape):
+    def __init__(self, views, world_from_ijk, center, step, E, pdf, mu, shape, pixels=None):
         self.step = float(step)
         self.shape = shape
         self.W_from_ijk = torch.tensor(np.asarray(world_from_ijk, np.float64))
@@ -87,11 +110,20 @@ class DenseDRR:
         self.M = len(mu) // len(E)
         self.mu = torch.tensor(np.asarray(mu, np.float64).reshape(len(E), self.M))
         self.views = []
-        for view, src in views:
-            r, rays, alphas, wts = _lattice(view, step)
+        # the alpha walk of all views' rays in one pass
+        setups = [_ray_setup(view, step, pixels) for view, _ in views]
+        starts = np.cumsum([0] + [len(st[1]) for st in setups])
+        cat = [np.concatenate([st[k] for st in setups]) for k in range(1, 5)]
+        rays_all, alphas_all, wts_all = _walk(*cat, step)
+        for i, ((view, src), st) in enumerate(zip(views, setups)):
+            sel = (rays_all >= starts[i]) & (rays_all < starts[i + 1])
+            rays, alphas, wts = rays_all[sel] - starts[i], alphas_all[sel], wts_all[sel]
+            r = st[0].astype(np.float64)
             x = np.asarray(src, np.float64)[None] + alphas[:, None] * r[rays]
-            self.views.append(dict(n=view.width * view.height, hw=(view.height, view.width),
-                                   rays=torch.tensor(rays), x=torch.tensor(x), wt=torch.tensor(wts)))
+            n = view.width * view.height if pixels is None else len(pixels)
+            hw = (view.height, view.width) if pixels is None else (len(pixels),)
+            self.views.append(dict(n=n, hw=hw, rays=torch.tensor(rays), x=torch.tensor(x),
+                                   wt=torch.tensor(wts)))
 
     def render(self, density, onehot, theta, extra_ad=None):
         """density [ni,nj,nk] float64, onehot [M,ni,nj,nk] float64, theta [6] float64;
