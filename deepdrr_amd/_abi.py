@@ -10,7 +10,7 @@ import os
 import threading
 from pathlib import Path
 
-from ._abi_types import BatchTuning, Job, MCParams, Scene, View
+from ._abi_types import BatchTuning, Job, MCParams, MeshPrim, Scene, View
 
 LIB_PATH = Path(__file__).resolve().parent / "libdeepdrr_amd.so"
 
@@ -46,6 +46,11 @@ SIGNATURES = {
     "drr_adjoint": (C.c_int, [_P, _P, C.POINTER(View), _P, _P, _P, _P, _P, _P, _P]),
     "drr_mc_scatter": (C.c_int, [_P, _P, C.POINTER(MCParams), C.c_int64, C.c_int64, C.c_uint64,
                                  _P, _P, _P, _P]),
+    "drr_mesh_create": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.POINTER(MeshPrim), C.c_int, C.c_int,
+                                  C.c_int, C.POINTER(_P)]),
+    "drr_mesh_destroy": (C.c_int, [_P]),
+    "drr_project_meshes": (C.c_int, [_P, _P, C.POINTER(View), _P, _P, C.c_float, _P, _P, _P, _P, _P]),
+    "drr_mesh_rays": (C.c_int, [_P, _P, C.POINTER(View), _P, C.c_float, _P, _P, _P, C.POINTER(C.c_int), _P]),
 }
 
 

@@ -8,6 +8,9 @@ import numpy as np
 DRR_MAX_VOLUMES = 4
 DRR_MAX_MATERIALS = 8
 DRR_MAX_BINS = 256
+DRR_MESH_MAX_LAYERS = 4
+DRR_MESH_MAX_HITS = 128
+DRR_MESH_RAY_HITS = 256
 
 
 class ViewVolume(C.Structure):
@@ -49,6 +52,16 @@ class Job(C.Structure):
     ]
 
 
+class MeshPrim(C.Structure):
+    _fields_ = [
+        ("material", C.c_int32),
+        ("density", C.c_float),
+        ("additive", C.c_int32),
+        ("subtractive", C.c_int32),
+        ("layer", C.c_int32),
+    ]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [
         ("tile_w", C.c_int32),
@@ -87,15 +100,16 @@ class MCParams(C.Structure):
     ]
 
 
-def make_view(world_from_index, width, height, max_ray_length, vols) -> View:
+def make_view(world_from_index, width, height, max_ray_length, vols, allow_empty=False) -> View:
     """vols: list of dicts with ijk_from_world(12), source_ijk(3), bounds_lo(3),
-    bounds_hi(3), priority, enabled."""
+    bounds_hi(3), priority, enabled (none only with ``allow_empty``, a scene of
+    meshes alone)."""
     v = View()
     v.world_from_index[:] = [float(x) for x in np.asarray(world_from_index, np.float32).reshape(9)]
     v.max_ray_length = float(np.float32(max_ray_length))
     v.width = int(width)
     v.height = int(height)
-    if not 1 <= len(vols) <= DRR_MAX_VOLUMES:
+    if not (0 if allow_empty else 1) <= len(vols) <= DRR_MAX_VOLUMES:
         raise ValueError(f"1..{DRR_MAX_VOLUMES} volumes per view are supported, got {len(vols)}")
     v.n_volumes = len(vols)
     for i, d in enumerate(vols):

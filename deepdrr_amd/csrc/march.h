@@ -441,18 +441,30 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
     for (int m = 0; m < M; m++) ad[m] /= 10.0f;
 }
 
+// Mesh terms of a ray for march_multi: `inside(alpha)` says whether a
+// subtractive mesh hides the sample at alpha (called with increasing alpha,
+// project_kernel.cu:498-521) and `add(ad)` adds the additive mesh area
+// densities after the step scaling (569-579).  NoMesh is the scene without
+// meshes.
+struct NoMesh {
+    __device__ __forceinline__ bool inside(float) { return false; }
+    template <int M>
+    __device__ __forceinline__ void add(float (&)[M]) {}
+};
+
 // General march for 1..V volumes with priorities and the label cache shared
 // across volumes (project_kernel.cu:394-553), literal step by step.
-template <int V, int M>
+template <int V, int M, typename Mesh = NoMesh>
 __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx, float step,
-                                            int atten, int air, float (&ad)[M]) {
+                                            int atten, int air, float (&ad)[M], Mesh mesh = Mesh()) {
     float rx, ry, rz, L;
     ray_dir(vd.W, udx, vdx, rx, ry, rz, L);
     float minA = L, maxA = 0.0f;
 #pragma unroll
     for (int m = 0; m < M; m++) ad[m] = 0.0f;
-    float ri[V][3], mnv[V], mxv[V];
-    bool trace[V];
+    constexpr int VA = V > 0 ? V : 1;  // array extent (V = 0: meshes only)
+    float ri[VA][3], mnv[VA], mxv[VA];
+    bool trace[VA];
 #pragma unroll
     for (int i = 0; i < V; i++) {
         ri[i][0] = ri[i][1] = ri[i][2] = 0.0f;
@@ -476,8 +488,8 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
     uint32_t l0 = 0, l1 = 0;
     float alpha = minA;
     for (int t = 0; t < N; t++) {
-        float p[V][3];
-        float seg[V][M];
+        float p[VA][3];
+        float seg[VA][M];
 #pragma unroll
         for (int i = 0; i < V; i++) {
 #pragma unroll
@@ -519,7 +531,9 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
                 n_at += 1;
             }
         }
-        if (0 == n_at) {
+        if (mesh.inside(alpha)) {
+            // the volumes (and the outside air) are hidden inside a mesh
+        } else if (0 == n_at) {
             if (atten) set_m<M>(ad, air, add_air(get_m<M>(ad, air)));
         } else {
             float weight = 1.0f / ((float)n_at);
@@ -551,6 +565,7 @@ __device__ __forceinline__ void march_multi(const ViewDev &vd, int udx, int vdx,
     }
 #pragma unroll
     for (int m = 0; m < M; m++) ad[m] *= step;
+    mesh.add(ad);
 #pragma unroll
     for (int m = 0; m < M; m++) ad[m] /= 10.0f;
 }

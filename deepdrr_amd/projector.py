@@ -7,9 +7,10 @@ validation (398-582), ``initialize`` (1395-1717), ``project`` / ``__call__``
 
 The volume path runs on hand-written HIP kernels through the C ABI
 (``libdeepdrr_amd.so``); PyTorch only provides device buffers.  Mesh
-rendering (OpenGL depth peeling) is not part of this engine: passing a
-non-``Volume`` raises ``ValueError`` as the reference does for unknown
-renderables.
+renderables (``deepdrr_amd.mesh.Mesh``) are ray-cast on the GPU
+(``drr_project_meshes``) in place of the reference's OpenGL depth peeling
+(projector.py:1055-1330); other renderables raise ``ValueError`` as in the
+reference.
 """
 from __future__ import annotations
 
@@ -24,6 +25,7 @@ import numpy as np
 from . import _abi, geo, utils
 from ._abi_types import Scene, make_view
 from .material import Material, attenuation_table
+from .mesh import DRRMaterial, Mesh, mesh_arrays
 from .spectral_data import get_spectrum, spectrum_tables
 from .volume import Volume
 
@@ -67,10 +69,10 @@ def world_from_index_rows(proj: geo.CameraProjection, sdd: float) -> np.ndarray:
 
 
 def build_view(volumes: Sequence[Volume], priorities: Sequence[int], proj: geo.CameraProjection,
-               max_ray_length: float, sdd: float = 1.0):
+               max_ray_length: float, sdd: float = 1.0, allow_empty: bool = False):
     w, h = proj.intrinsic.sensor_size
     vols = [volume_view_params(v, proj, p) for v, p in zip(volumes, priorities)]
-    return make_view(world_from_index_rows(proj, sdd), w, h, max_ray_length, vols)
+    return make_view(world_from_index_rows(proj, sdd), w, h, max_ray_length, vols, allow_empty=allow_empty)
 
 
 def volume_arrays(vol: Volume, all_materials: List[str]):
@@ -121,11 +123,21 @@ class Projector:
         self.mesh_layers = mesh_layers
         volume = utils.listify(volume)
         self.volumes: List[Volume] = []
+        self.meshes: List[Mesh] = []
+        self.primitives = []
         for _vol in volume:
             if isinstance(_vol, Volume):
                 self.volumes.append(_vol)
+            elif isinstance(_vol, Mesh):
+                self.meshes.append(_vol)
+                for prim in _vol.mesh.primitives:
+                    if isinstance(prim.material, DRRMaterial):
+                        self.primitives.append(prim)
+                    else:
+                        raise ValueError(f"unrecognized material type: {type(_vol)}.")
             else:
                 raise ValueError(f"unrecognized Renderable type: {type(_vol)}.")
+        self.mesh_additive_enabled = len(self.meshes) > 0
         if len(self.volumes) > 4:
             raise ValueError("at most 4 volumes per projector are supported")
         if priorities is None:
@@ -181,10 +193,16 @@ class Projector:
         self.max_mesh_hits = max_mesh_hits
         if self.max_mesh_hits < 4 or self.max_mesh_hits % 4 != 0:
             raise ValueError("max_mesh_depth must be a multiple of 4 and >= 4")
+        if self.meshes and self.max_mesh_hits > 128:
+            raise ValueError("at most 128 mesh hits per layer are supported")
+        if self.meshes and not 1 <= self.mesh_layers <= 4:
+            raise ValueError("mesh_layers must be in [1, 4]")
 
         all_mats = []
         for _vol in self.volumes:
             all_mats.extend(list(_vol.materials[0].keys()))
+        for prim in self.primitives:
+            all_mats.append(prim.material.drrMatName)
         if attenuate_outside_volume:
             all_mats.append("air")
         self.all_materials = sorted(set(all_mats))
@@ -212,6 +230,8 @@ class Projector:
         self.initialized = False
         self._ctx = None
         self._vols = []
+        self._mesh = None      # drr_mesh handle of the meshes' current poses
+        self._mesh_key = None
 
     # -- properties (projector.py:584-626) --------------------------------
     @property
@@ -290,6 +310,10 @@ class Projector:
                                 if self.collected_energy else None)
 
     def _release(self) -> None:
+        if self._mesh is not None:
+            _abi.lib().drr_mesh_destroy(self._mesh)
+            self._mesh = None
+            self._mesh_key = None
         for h in self._vols:
             _abi.lib().drr_volume_destroy(h)
         self._vols = []
@@ -336,17 +360,44 @@ class Projector:
     def view_for(self, proj: geo.CameraProjection):
         sdd = self.source_to_detector_distance
         return build_view(self.volumes, self.priorities, proj, self.max_ray_length,
-                          sdd if sdd > 0 else 1.0)
+                          sdd if sdd > 0 else 1.0, allow_empty=bool(self.meshes))
+
+    def _mesh_handle(self):
+        """drr_mesh of the meshes at their current poses and visibility (rebuilt
+        when either changes; the reference re-poses its pyrender nodes per
+        view, projector.py:855-857)."""
+        key = tuple((id(m), m.enabled, np.asarray(m.world_from_ijk.data).tobytes()) for m in self.meshes)
+        if key != self._mesh_key:
+            if self._mesh is not None:
+                _abi.lib().drr_mesh_destroy(self._mesh)
+                self._mesh = None
+            verts, tris, tprim, prims, n_prims = mesh_arrays(self.meshes, self.all_materials, self.mesh_layers)
+            h = C.c_void_p()
+            _abi.call("drr_mesh_create", self._ctx, verts.ctypes.data, len(verts), tris.ctypes.data,
+                      tprim.ctypes.data, len(tris), prims, n_prims, self.mesh_layers, self.max_mesh_hits,
+                      C.byref(h))
+            self._mesh = h
+            self._mesh_key = key
+        return self._mesh
+
+    def _launch_view(self, proj: geo.CameraProjection, intensity, photon_prob, area_density, solid_angle, stream):
+        view = self.view_for(proj)
+        if self.meshes:
+            src = np.ascontiguousarray(np.asarray(proj.center_in_world), np.float32)
+            sdd = self.source_to_detector_distance
+            _abi.call("drr_project_meshes", self._ctx, self._vol_array, C.byref(view), self._mesh_handle(),
+                      src.ctypes.data, C.c_float(sdd), _abi.ptr(intensity), _abi.ptr(photon_prob),
+                      _abi.ptr(area_density), _abi.ptr(solid_angle), stream)
+        else:
+            _abi.call("drr_project", self._ctx, self._vol_array, C.byref(view), _abi.ptr(intensity),
+                      _abi.ptr(photon_prob), _abi.ptr(area_density), _abi.ptr(solid_angle), stream)
 
     def _render_single(self, proj: geo.CameraProjection):
         import torch
 
         self.initialize_output_arrays(proj.intrinsic.sensor_size)
-        view = self.view_for(proj)
         stream = _abi.stream_ptr(torch.cuda.current_stream(self._torch_device))
-        _abi.call("drr_project", self._ctx, self._vol_array, C.byref(view),
-                  _abi.ptr(self.intensity_gpu), _abi.ptr(self.photon_prob_gpu), None,
-                  _abi.ptr(self.solid_angle_gpu), stream)
+        self._launch_view(proj, self.intensity_gpu, self.photon_prob_gpu, None, self.solid_angle_gpu, stream)
         intensity = to_host(self.intensity_gpu)
         photon_prob = to_host(self.photon_prob_gpu)
         if self.collected_energy:
@@ -399,6 +450,9 @@ class Projector:
         """
         from .batch import plan_for_projector
 
+        if self.meshes:
+            # meshes are ray-cast per view; the batch plan has no mesh pass
+            return self.project(*camera_projections)
         camera_projections = self._prepare_project(camera_projections)
         sizes = {tuple(p.intrinsic.sensor_size) for p in camera_projections}
         # noise and neglog run on the device (one D2H copy) when every image has
@@ -473,7 +527,5 @@ class Projector:
         AD = torch.empty((M, h, w), dtype=torch.float32, device=self._torch_device)
         if not hasattr(self, "max_ray_length"):
             self.max_ray_length = self.source_to_detector_distance * 4
-        view = self.view_for(proj)
-        _abi.call("drr_project", self._ctx, self._vol_array, C.byref(view), _abi.ptr(I), _abi.ptr(P),
-                  _abi.ptr(AD), None, _abi.stream_ptr(torch.cuda.current_stream(self._torch_device)))
+        self._launch_view(proj, I, P, AD, None, _abi.stream_ptr(torch.cuda.current_stream(self._torch_device)))
         return to_host(AD), to_host(I), to_host(P)

@@ -39,6 +39,7 @@ extern "C" {
 typedef struct drr_ctx_s drr_ctx;
 typedef struct drr_volume_s drr_volume;
 typedef struct drr_batch_s drr_batch;
+typedef struct drr_mesh_s drr_mesh;
 
 /* Per-volume part of one view (projector.py:802-831, 1626-1637). */
 typedef struct {
@@ -128,6 +129,64 @@ int drr_volume_download(drr_volume *vol, float *density, uint8_t *labels);
 int drr_project(drr_ctx *ctx, const drr_volume *const *vols, const drr_view_t *view,
                 float *intensity, float *photon_prob, float *area_density,
                 float *solid_angle, void *stream);
+
+/* ---- meshes ------------------------------------------------------------------ */
+
+#define DRR_MESH_MAX_LAYERS 4
+#define DRR_MESH_MAX_HITS 128
+#define DRR_MESH_RAY_HITS 256
+
+/* One primitive's material (pyrenderdrr/material.py DRRMaterial: density,
+ * additive, subtractive, layer, drrMatName). */
+typedef struct {
+    int32_t material;    /* global material index (< n_materials) */
+    float density;       /* g/cm^3 */
+    int32_t additive;    /* adds density x chord length to its material */
+    int32_t subtractive; /* hides the volumes, and the additive meshes of lower
+                            layers, inside it */
+    int32_t layer;       /* 0 <= layer < n_layers */
+} drr_mesh_prim_t;
+
+/* Triangle meshes of a scene, in world coordinates (mm), ray-cast on the GPU
+ * in place of the reference's OpenGL depth peeling (projector.py:1055-1330)
+ * and its peel post-processing (peel_postprocess_kernel.cu:15-235).
+ * vertices: [n_vertices][3] float; triangles: [n_triangles][3] vertex
+ * indices, counter-clockwise seen from outside; triangle_prim: [n_triangles]
+ * index into prims.  n_layers: 1..DRR_MESH_MAX_LAYERS (Projector mesh_layers);
+ * max_hits: subtractive hits kept per ray and layer, a multiple of 4 in
+ * [4, DRR_MESH_MAX_HITS] (max_mesh_hits).  A bounding-volume hierarchy is
+ * built on the host and uploaded; the handle owns it. */
+int drr_mesh_create(drr_ctx *ctx, const float *vertices, int n_vertices, const int32_t *triangles,
+                    const int32_t *triangle_prim, int n_triangles, const drr_mesh_prim_t *prims,
+                    int n_prims, int n_layers, int max_hits, drr_mesh **out);
+int drr_mesh_destroy(drr_mesh *mesh);
+
+/* drr_project with meshes (project_kernel.cu:363-375, 498-521, 569-579):
+ * view->n_volumes may be 0 (meshes only).  source_world: 3 host floats, the
+ * source in world coordinates (the rays' origin; alpha is the distance from
+ * it along the unit world ray).  source_to_detector_distance (mm, <= 0 if
+ * unknown) sets the hit range the reference's peel post-processing keeps
+ * (alpha < 2 SDD - 0.001) and the far plane of the additive pass (4 SDD).
+ * Outputs as drr_project.  The per-ray mesh terms live in buffers the mesh
+ * handle owns, so calls that share a handle must not overlap in time. */
+int drr_project_meshes(drr_ctx *ctx, const drr_volume *const *vols, const drr_view_t *view,
+                       const drr_mesh *mesh, const float *source_world, float source_to_detector_distance,
+                       float *intensity, float *photon_prob, float *area_density, float *solid_angle,
+                       void *stream);
+
+/* The per-ray mesh terms drr_project_meshes uses, for inspection:
+ * intervals: device float [height][width][max_intervals][2] with
+ * max_intervals = n_layers * max_hits / 2, the sorted disjoint ranges
+ * (a, b] of alpha hidden by subtractive meshes (any layer inside);
+ * counts: device int32 [height][width]; additive: device float
+ * [n_layers * n_unique][height][width], per (layer, unique material in
+ * ascending global index) the clamped additive area density in mm g/cm^3
+ * (0 where the ray's entries and exits do not balance); n_unique (host int,
+ * may be NULL) receives the number of unique materials of additive
+ * primitives. */
+int drr_mesh_rays(drr_ctx *ctx, const drr_mesh *mesh, const drr_view_t *view, const float *source_world,
+                  float source_to_detector_distance, float *intervals, int32_t *counts, float *additive,
+                  int *n_unique, void *stream);
 
 /* ---- batched forward (one persistent launch, hipGraph) ------------------- */
 

@@ -39,6 +39,20 @@ def lib():
         l.ora_project_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(View), C.POINTER(Scene),
                                        C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        from deepdrr_amd._abi_types import MeshPrim
+
+        l.ora_project_mesh.restype = C.c_int
+        l.ora_project_mesh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(View), C.POINTER(Scene),
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.POINTER(MeshPrim), C.c_int, C.c_int, C.c_int,
+                                       C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int]
+        l.ora_mesh_rays.restype = C.c_int
+        l.ora_mesh_rays.argtypes = [C.POINTER(View), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.POINTER(MeshPrim), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
+        l.ora_ray_triangle.restype = C.c_int
+        l.ora_ray_triangle.argtypes = [C.c_void_p] * 5 + [C.POINTER(C.c_float), C.POINTER(C.c_int)]
         l.ora_solid_angle.restype = None
         l.ora_solid_angle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         _lib = l
@@ -84,6 +98,66 @@ def project(volumes, view: View, scene: Scene, energies, pdf, mu, n_threads: int
     if rc != 0:
         raise ValueError("ora_project rejected its arguments")
     return I, P, AD
+
+
+def project_mesh(volumes, view: View, scene: Scene, energies, pdf, mu, mesh, n_layers: int, max_hits: int,
+                 source_world, sdd: float, rows=None, n_threads: int | None = None):
+    """The transcription with meshes.  ``mesh`` = (vertices float32 [n,3],
+    triangles int32 [m,3], triangle primitive int32 [m], MeshPrim table,
+    n_prims) as ``deepdrr_amd.mesh.mesh_arrays`` returns; ``volumes`` may be
+    empty (view.n_volumes == 0)."""
+    n_threads = n_threads or default_threads()
+    V = view.n_volumes
+    dens = [np.ascontiguousarray(d, dtype=np.float32) for d, _ in volumes[:V]]
+    labs = [np.ascontiguousarray(l, dtype=np.uint8) for _, l in volumes[:V]]
+    dptr = (C.c_void_p * max(1, V))(*[d.ctypes.data for d in dens])
+    lptr = (C.c_void_p * max(1, V))(*[l.ctypes.data for l in labs])
+    dims = np.array([s for d in dens for s in d.shape[::-1]] or [0], dtype=np.int32)
+    W, M = view.width, scene.n_materials
+    rsel = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    H = view.height if rsel is None else len(rsel)
+    I = np.zeros((H, W), np.float32)
+    P = np.zeros((H, W), np.float32)
+    AD = np.zeros((M, H, W), np.float32)
+    e = np.ascontiguousarray(energies, np.float32)
+    p = np.ascontiguousarray(pdf, np.float32)
+    m = np.ascontiguousarray(mu, np.float32)
+    verts, tris, tprim, prims, n_prims = mesh
+    src = np.ascontiguousarray(source_world, np.float32)
+    rc = lib().ora_project_mesh(dptr, lptr, dims.ctypes.data, C.byref(view), C.byref(scene), len(e),
+                                e.ctypes.data, p.ctypes.data, m.ctypes.data, verts.ctypes.data, tris.ctypes.data,
+                                tprim.ctypes.data, len(tris), prims, n_prims, n_layers, max_hits, src.ctypes.data,
+                                float(sdd), None if rsel is None else rsel.ctypes.data,
+                                0 if rsel is None else len(rsel), I.ctypes.data, P.ctypes.data, AD.ctypes.data,
+                                int(n_threads))
+    if rc != 0:
+        raise ValueError("ora_project_mesh rejected its arguments")
+    return I, P, AD
+
+
+def mesh_rays(view: View, mesh, n_layers: int, max_hits: int, n_materials: int, source_world, sdd: float):
+    """Per-ray mesh terms: intervals [H,W,K,2], counts [H,W], additive
+    [n_layers * n_unique, H, W] and n_unique (K = n_layers * max_hits / 2)."""
+    verts, tris, tprim, prims, n_prims = mesh
+    H, W = view.height, view.width
+    K = n_layers * max_hits // 2
+    iv = np.zeros((H, W, K, 2), np.float32)
+    cnt = np.zeros((H, W), np.int32)
+    add = np.zeros((n_layers * 8, H, W), np.float32)
+    src = np.ascontiguousarray(source_world, np.float32)
+    nu = lib().ora_mesh_rays(C.byref(view), verts.ctypes.data, tris.ctypes.data, tprim.ctypes.data, len(tris),
+                             prims, n_prims, n_layers, max_hits, n_materials, src.ctypes.data, float(sdd),
+                             iv.ctypes.data, cnt.ctypes.data, add.ctypes.data)
+    return iv, cnt, add[: n_layers * nu], nu
+
+
+def ray_triangle(o, d, v0, v1, v2):
+    """(hit, t, entry) of the oracle's FP32 Moller-Trumbore test."""
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (o, d, v0, v1, v2)]
+    t = C.c_float()
+    ent = C.c_int()
+    hit = lib().ora_ray_triangle(*[a.ctypes.data for a in arrs], C.byref(t), C.byref(ent))
+    return bool(hit), t.value, bool(ent.value)
 
 
 def solid_angle(world_from_index, width: int, height: int) -> np.ndarray:

@@ -24,6 +24,10 @@
  *     accumulation with 1/n and end halves  522-549
  *   air suffix, step scale, /10             557-584
  *   spectral Beer-Lambert sum               637-646
+ *   meshes (ora_project_mesh): hit lists    363-375
+ *     inside-mesh test per step             498-521
+ *     additive mesh densities               569-579
+ *     (the hit lists themselves: oracle/mesh_hits.c)
  *
  * Texture semantics, restated exactly (the reference samples CUDA textures):
  *   labels  (point, clamp): corner d of the cell is L[clamp(floor(p + d))]
@@ -45,6 +49,7 @@
 #include <string.h>
 
 #include "../include/deepdrr_amd.h"
+#include "mesh_hits.h"
 
 #define AIR_DENSITY 0.1129 /* double literal, as project_kernel.cu:6 */
 
@@ -85,9 +90,18 @@ static inline int floor_to_int(float b) {
 }
 
 /* One ray.  ad[] receives the final area densities (g/cm^2). */
+typedef struct {
+    const ora_mesh *mesh;
+    float src[3]; /* source, world */
+    float sdd;
+    int n_unique;
+    int unique_material[DRR_MAX_MATERIALS];
+} ora_mesh_scene;
+
 static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_scene_t *sc,
                     int udx, int vdx, int n_bins, const float *energies, const float *pdf,
-                    const float *mu, float *out_intensity, float *out_photon_prob, float *ad) {
+                    const float *mu, float *out_intensity, float *out_photon_prob, float *ad,
+                    const ora_mesh_scene *ms) {
     const int V = view->n_volumes;
     const int M = sc->n_materials;
     const float step = sc->step;
@@ -107,6 +121,24 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
     float minAlpha = ray_length;
     float maxAlpha = 0;
     for (int m = 0; m < M; m++) ad[m] = 0.0f;
+
+    /* the meshes' per-ray terms (what OpenGL and the peel kernels hand to the
+     * reference kernel), and its per-layer hit cursors (363-375) */
+    ora_mesh_ray mr;
+    int NL = 0, MH = 0;
+    int mesh_hit_depth[DRR_MESH_MAX_LAYERS], mesh_hit_index[DRR_MESH_MAX_LAYERS];
+    if (ms) {
+        const float dir[3] = {rx, ry, rz};
+        mr.n_unique = ms->n_unique;
+        for (int i = 0; i < ms->n_unique; i++) mr.unique_material[i] = ms->unique_material[i];
+        ora_mesh_ray_terms(ms->mesh, ms->src, dir, ms->sdd, &mr);
+        NL = ms->mesh->n_layers;
+        MH = ms->mesh->max_hits;
+        for (int j = 0; j < NL; j++) {
+            mesh_hit_depth[j] = 0;
+            mesh_hit_index[j] = 0;
+        }
+    }
 
     float minA_v[DRR_MAX_VOLUMES], maxA_v[DRR_MAX_VOLUMES];
     int do_trace[DRR_MAX_VOLUMES];
@@ -215,7 +247,21 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
             }
         }
 
-        if (0 == n_at) {
+        int inside_mesh = 0;
+        for (int j = 0; j < NL; j++) {
+            if (mr.layer_valid[j] == 0) continue;
+            while (mesh_hit_index[j] < MH && mr.hit_facing[j * MH + mesh_hit_index[j]] != 0 &&
+                   mr.hit_alpha[j * MH + mesh_hit_index[j]] < alpha) {
+                mesh_hit_depth[j] += mr.hit_facing[j * MH + mesh_hit_index[j]];
+                mesh_hit_index[j] += 1;
+            }
+        }
+        for (int j = 0; j < NL; j++)
+            if (mesh_hit_depth[j] > 0) inside_mesh = 1;
+
+        if (inside_mesh) {
+            /* the volumes (and the outside air) are hidden inside a mesh */
+        } else if (0 == n_at) {
             if (sc->attenuate_outside)
                 ad[sc->air_index] = (float)((double)ad[sc->air_index] + AIR_DENSITY);
         } else {
@@ -254,6 +300,12 @@ static void ora_ray(const ora_volume *vols, const drr_view_t *view, const drr_sc
                                     (double)((ray_length - maxAlpha) / step) * AIR_DENSITY);
     }
     for (int m = 0; m < M; m++) ad[m] *= step;
+    if (ms) {
+        for (int i = 0; i < mr.n_unique; i++)
+            for (int j = 0; j < NL; j++)
+                if (fabsf(mr.additive_facing[j * mr.n_unique + i]) < 0.00001f)
+                    ad[mr.unique_material[i]] += fmaxf(mr.additive[j * mr.n_unique + i], 0.0f);
+    }
     for (int m = 0; m < M; m++) ad[m] /= 10.0f;
 
     float intensity = 0.0f, photon_prob = 0.0f;
@@ -280,8 +332,8 @@ static int ora_project_impl(const float *const *density, const uint8_t *const *l
                             const drr_view_t *view, const drr_scene_t *scene, int n_bins,
                             const float *energies, const float *pdf, const float *mu, const int *rows,
                             int n_rows, float *intensity, float *photon_prob, float *area_density,
-                            int n_threads) {
-    if (view->n_volumes < 1 || view->n_volumes > DRR_MAX_VOLUMES) return -1;
+                            int n_threads, const ora_mesh_scene *ms) {
+    if (view->n_volumes < (ms ? 0 : 1) || view->n_volumes > DRR_MAX_VOLUMES) return -1;
     if (scene->n_materials < 1 || scene->n_materials > DRR_MAX_MATERIALS) return -1;
     ora_volume vols[DRR_MAX_VOLUMES];
     for (int v = 0; v < view->n_volumes; v++) {
@@ -303,7 +355,7 @@ static int ora_project_impl(const float *const *density, const uint8_t *const *l
         int vdx = rows ? rows[r] : r;
         float ad[DRR_MAX_MATERIALS];
         ora_ray(vols, view, scene, udx, vdx, n_bins, energies, pdf, mu, &intensity[idx],
-                &photon_prob[idx], ad);
+                &photon_prob[idx], ad, ms);
         if (area_density)
             for (int m = 0; m < M; m++) area_density[(long)m * npix + idx] = ad[m];
     }
@@ -315,7 +367,7 @@ int ora_project(const float *const *density, const uint8_t *const *labels, const
                 const float *energies, const float *pdf, const float *mu, float *intensity,
                 float *photon_prob, float *area_density, int n_threads) {
     return ora_project_impl(density, labels, dims, view, scene, n_bins, energies, pdf, mu, NULL, 0,
-                            intensity, photon_prob, area_density, n_threads);
+                            intensity, photon_prob, area_density, n_threads, NULL);
 }
 
 int ora_project_rows(const float *const *density, const uint8_t *const *labels, const int *dims,
@@ -325,7 +377,122 @@ int ora_project_rows(const float *const *density, const uint8_t *const *labels, 
                      int n_threads) {
     if (!rows || n_rows < 1) return -1;
     return ora_project_impl(density, labels, dims, view, scene, n_bins, energies, pdf, mu, rows, n_rows,
-                            intensity, photon_prob, area_density, n_threads);
+                            intensity, photon_prob, area_density, n_threads, NULL);
+}
+
+/*
+ * Project one view with meshes (rows: NULL for all, as ora_project_rows).
+ * view->n_volumes may be 0.  The mesh's unique materials are those of its
+ * additive primitives, in ascending global index.
+ */
+int ora_project_mesh(const float *const *density, const uint8_t *const *labels, const int *dims,
+                     const drr_view_t *view, const drr_scene_t *scene, int n_bins, const float *energies,
+                     const float *pdf, const float *mu, const float *vertices, const int32_t *triangles,
+                     const int32_t *tri_prim, int n_triangles, const drr_mesh_prim_t *prims, int n_prims,
+                     int n_layers, int max_hits, const float *source_world, float sdd, const int *rows,
+                     int n_rows, float *intensity, float *photon_prob, float *area_density, int n_threads) {
+    if (n_layers < 1 || n_layers > DRR_MESH_MAX_LAYERS || max_hits < 4 || max_hits > DRR_MESH_MAX_HITS ||
+        max_hits % 4 != 0)
+        return -1;
+    ora_mesh mesh = {vertices, triangles, tri_prim, n_triangles, prims, n_prims, n_layers, max_hits};
+    ora_mesh_scene ms;
+    ms.mesh = &mesh;
+    for (int a = 0; a < 3; a++) ms.src[a] = source_world[a];
+    ms.sdd = sdd;
+    ms.n_unique = 0;
+    for (int m = 0; m < scene->n_materials; m++) {
+        int used = 0;
+        for (int p = 0; p < n_prims; p++)
+            if (prims[p].additive && prims[p].material == m) used = 1;
+        if (used) ms.unique_material[ms.n_unique++] = m;
+    }
+    return ora_project_impl(density, labels, dims, view, scene, n_bins, energies, pdf, mu, rows, n_rows,
+                            intensity, photon_prob, area_density, n_threads, &ms);
+}
+
+/* Per-ray mesh terms of one view for inspection: hidden alpha ranges
+ * (a, b] merged over the layers ([H][W][n_layers * max_hits / 2][2], count
+ * [H][W]) and the clamped additive terms ([n_layers * n_unique][H][W], 0 where
+ * the facing sum is not 0).  Returns n_unique. */
+int ora_mesh_rays(const drr_view_t *view, const float *vertices, const int32_t *triangles,
+                  const int32_t *tri_prim, int n_triangles, const drr_mesh_prim_t *prims, int n_prims,
+                  int n_layers, int max_hits, int n_materials, const float *source_world, float sdd,
+                  float *intervals, int32_t *counts, float *additive) {
+    ora_mesh mesh = {vertices, triangles, tri_prim, n_triangles, prims, n_prims, n_layers, max_hits};
+    int n_unique = 0, uniq[DRR_MAX_MATERIALS];
+    for (int m = 0; m < n_materials; m++) {
+        int used = 0;
+        for (int p = 0; p < n_prims; p++)
+            if (prims[p].additive && prims[p].material == m) used = 1;
+        if (used) uniq[n_unique++] = m;
+    }
+    const int W = view->width, H = view->height, K = n_layers * max_hits / 2;
+    const long npix = (long)W * H;
+    for (long idx = 0; idx < npix; idx++) {
+        int vdx = (int)(idx / W), udx = (int)(idx % W);
+        const float *Wm = view->world_from_index;
+        float u = (float)((double)(float)udx + 0.5);
+        float v = (float)((double)(float)vdx + 0.5);
+        float rx = u * Wm[0] + v * Wm[1] + Wm[2];
+        float ry = u * Wm[3] + v * Wm[4] + Wm[5];
+        float rz = u * Wm[6] + v * Wm[7] + Wm[8];
+        float L = sqrtf(rx * rx + ry * ry + rz * rz);
+        float inv = 1.0f / L;
+        const float dir[3] = {rx * inv, ry * inv, rz * inv};
+        ora_mesh_ray mr;
+        mr.n_unique = n_unique;
+        for (int i = 0; i < n_unique; i++) mr.unique_material[i] = uniq[i];
+        ora_mesh_ray_terms(&mesh, source_world, dir, sdd, &mr);
+        /* inside(alpha) = some layer's depth (sum of facings of hits < alpha) > 0 */
+        float ts[DRR_MESH_MAX_LAYERS * DRR_MESH_MAX_HITS];
+        int nt = 0;
+        for (int j = 0; j < n_layers; j++)
+            if (mr.layer_valid[j])
+                for (int k = 0; k < max_hits && mr.hit_facing[j * max_hits + k] != 0; k++)
+                    ts[nt++] = mr.hit_alpha[j * max_hits + k];
+        /* sort the distinct hit alphas; the state is constant between them */
+        for (int a = 1; a < nt; a++)
+            for (int b = a; b > 0 && ts[b - 1] > ts[b]; b--) {
+                float x = ts[b];
+                ts[b] = ts[b - 1];
+                ts[b - 1] = x;
+            }
+        int n_iv = 0, open = 0;
+        float start = 0.0f;
+        for (int k = 0; k < nt; k++) {
+            if (k > 0 && ts[k] == ts[k - 1]) continue;
+            /* state for alpha just above ts[k] */
+            int in = 0;
+            for (int j = 0; j < n_layers && !in; j++) {
+                if (!mr.layer_valid[j]) continue;
+                int d = 0;
+                for (int q = 0; q < max_hits && mr.hit_facing[j * max_hits + q] != 0; q++)
+                    if (mr.hit_alpha[j * max_hits + q] <= ts[k]) d += mr.hit_facing[j * max_hits + q];
+                if (d > 0) in = 1;
+            }
+            if (in && !open) {
+                start = ts[k];
+                open = 1;
+            } else if (!in && open) {
+                intervals[(idx * K + n_iv) * 2] = start;
+                intervals[(idx * K + n_iv) * 2 + 1] = ts[k];
+                n_iv++;
+                open = 0;
+            }
+        }
+        if (open) {
+            intervals[(idx * K + n_iv) * 2] = start;
+            intervals[(idx * K + n_iv) * 2 + 1] = INFINITY;
+            n_iv++;
+        }
+        counts[idx] = n_iv;
+        for (int j = 0; j < n_layers; j++)
+            for (int i = 0; i < n_unique; i++)
+                additive[(long)(j * n_unique + i) * npix + idx] =
+                    fabsf(mr.additive_facing[j * n_unique + i]) < 0.00001f ? fmaxf(mr.additive[j * n_unique + i], 0.0f)
+                                                                           : 0.0f;
+    }
+    return n_unique;
 }
 
 /* Solid angle of each pixel seen from the source (project_kernel.cu:14-133),

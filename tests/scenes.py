@@ -102,6 +102,55 @@ def many_case(n_vols, seed=0, atten=False, priorities=None) -> Case:
                      atten=atten, priorities=priorities)
 
 
+@dataclass
+class MeshCase:
+    """A scene of volumes (possibly none) and meshes."""
+    case: Case
+    mesh: tuple              # deepdrr_amd.mesh.mesh_arrays(...)
+    n_layers: int
+    max_hits: int
+    source: np.ndarray       # world, float32
+    sdd: float
+
+    def oracle(self, rows=None):
+        from oracle import ctrans
+
+        c = self.case
+        return ctrans.project_mesh(c.volumes, c.view, c.scene, c.E, c.pdf, c.mu, self.mesh, self.n_layers,
+                                   self.max_hits, self.source, self.sdd, rows=rows)
+
+    def oracle_rays(self):
+        from oracle import ctrans
+
+        return ctrans.mesh_rays(self.case.view, self.mesh, self.n_layers, self.max_hits,
+                                self.case.scene.n_materials, self.source, self.sdd)
+
+
+def make_mesh_case(name, vols: List[Volume], meshes, proj: geo.CameraProjection, sdd=1020.0, n_layers=2,
+                   max_hits=32, step=0.1, atten=False, max_ray_length=4080.0) -> MeshCase:
+    from deepdrr_amd.mesh import mesh_arrays
+
+    mats = set()
+    for v in vols:
+        mats |= set(v.materials[0].keys())
+    for m in meshes:
+        for prim in m.mesh.primitives:
+            mats.add(prim.material.drrMatName)
+    if atten:
+        mats.add("air")
+    mats = sorted(mats)
+    E, pdf = spectrum_tables(synthetic_120kv())
+    mu = attenuation_table(mats, E)
+    prios = [len(vols) - 1 - i for i in range(len(vols))]
+    view = P.build_view(vols, prios, proj, max_ray_length, allow_empty=True)
+    arrays = [P.volume_arrays(v, mats) for v in vols]
+    sc = Scene(step=step, n_materials=len(mats), attenuate_outside=1 if atten else 0,
+               air_index=mats.index("air") if atten else 0)
+    case = Case(name, arrays, view, sc, E, pdf, mu, mats)
+    return MeshCase(case, mesh_arrays(meshes, mats, n_layers), n_layers, max_hits,
+                    np.asarray(proj.center_in_world, np.float32), float(sdd))
+
+
 class GpuRunner:
     """Owns a drr_ctx and volume handles for one Case."""
 
@@ -139,6 +188,43 @@ class GpuRunner:
         torch.cuda.synchronize()
         out = (I.cpu().numpy(), Pp.cpu().numpy(), AD.cpu().numpy())
         return out + ((SA.cpu().numpy(),) if solid_angle else ())
+
+    def project_mesh(self, mc: MeshCase, want_rays=False):
+        """drr_project_meshes of the mesh case (this runner's volumes); with
+        ``want_rays`` also drr_mesh_rays' intervals, counts and additive terms."""
+        torch = self.torch
+        view = mc.case.view
+        H, W, M = view.height, view.width, self.case.scene.n_materials
+        dev = torch.device("cuda", 0)
+        verts, tris, tprim, prims, n_prims = mc.mesh
+        h = C.c_void_p()
+        self.abi.call("drr_mesh_create", self.ctx, verts.ctypes.data, len(verts), tris.ctypes.data,
+                      tprim.ctypes.data, len(tris), prims, n_prims, mc.n_layers, mc.max_hits, C.byref(h))
+        try:
+            I = torch.full((H, W), float("nan"), device=dev)
+            Pp = torch.full((H, W), float("nan"), device=dev)
+            AD = torch.full((M, H, W), float("nan"), device=dev)
+            src = np.ascontiguousarray(mc.source, np.float32)
+            self.abi.call("drr_project_meshes", self.ctx, self.varr, C.byref(view), h, src.ctypes.data,
+                          C.c_float(mc.sdd), self.abi.ptr(I), self.abi.ptr(Pp), self.abi.ptr(AD), None,
+                          self.abi.stream_ptr())
+            torch.cuda.synchronize()
+            out = (I.cpu().numpy(), Pp.cpu().numpy(), AD.cpu().numpy())
+            if want_rays:
+                K = mc.n_layers * mc.max_hits // 2
+                iv = torch.full((H, W, K, 2), float("nan"), device=dev)
+                cnt = torch.full((H, W), -1, dtype=torch.int32, device=dev)
+                add = torch.full((mc.n_layers * 8, H, W), float("nan"), device=dev)
+                nu = C.c_int()
+                self.abi.call("drr_mesh_rays", self.ctx, h, C.byref(view), src.ctypes.data, C.c_float(mc.sdd),
+                              self.abi.ptr(iv), self.abi.ptr(cnt), self.abi.ptr(add), C.byref(nu),
+                              self.abi.stream_ptr())
+                torch.cuda.synchronize()
+                out = out + (iv.cpu().numpy(), cnt.cpu().numpy(), add.cpu().numpy()[: mc.n_layers * nu.value],
+                             nu.value)
+            return out
+        finally:
+            self.abi.lib().drr_mesh_destroy(h)
 
     def close(self):
         for v in self.vols:
