@@ -45,7 +45,7 @@ SIGNATURES = {
     "drr_add_noise": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
     "drr_adjoint": (C.c_int, [_P, _P, C.POINTER(View), _P, _P, _P, _P, _P, _P, _P]),
     "drr_mc_scatter": (C.c_int, [_P, _P, C.POINTER(MCParams), C.c_int64, C.c_int64, C.c_uint64,
-                                 _P, _P, _P, _P]),
+                                 _P, _P, _P, _P, _P]),
     "drr_mesh_create": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.POINTER(MeshPrim), C.c_int, C.c_int,
                                   C.c_int, C.POINTER(_P)]),
     "drr_mesh_destroy": (C.c_int, [_P]),

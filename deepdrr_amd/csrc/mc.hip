@@ -13,9 +13,23 @@
 // Random numbers: RANECU (L'Ecuyer 1988, as in PENELOPE and MC-GPU), one
 // stream per history seeded from (seed, history index) by SplitMix64.
 //
+// Local majorants: one global majorant (the densest voxel's attenuation, here
+// a titanium rod) made a photon in soft tissue take ~155 flights, most of them
+// virtual.  The volume is cut into 8x8x8-voxel bricks, and each brick stores
+// the component-wise maxima of density x (photoelectric, Compton, Rayleigh)
+// coefficient over its voxels (k_mc_bricks, per call); its majorant at energy
+// E is maj_pe f_pe(E) + maj_c f_c(E) + maj_r f_r(E), which bounds every
+// voxel's attenuation in FP32 too (products and sums round monotonically).  A
+// flight draws its optical depth once and walks the bricks along the
+// direction (Amanatides-Woo), spending depth at each brick's own majorant;
+// where it runs out is the tentative collision, accepted as real with
+// probability mu(voxel) / mu(brick).  This is delta tracking with a piecewise
+// constant majorant, unbiased like the global one (Badal & Badano 2009 use a
+// global majorant; the local form is standard in Woodcock trackers).
+//
 // GPU mapping: every lane owns one history at a time and starts the next one
 // as soon as its photon is absorbed or scored (no waiting for the slowest
-// history of the wave); each loop iteration is one Woodcock flight.
+// history of the wave); each loop iteration is one flight.
 #include <hip/hip_runtime.h>
 
 #include "drr_internal.h"
@@ -120,28 +134,56 @@ struct Photon {
 
 constexpr float TWO_PI = 6.283185307179586f;
 
-// (E / 60 keV)^(-k) of the three processes from one log2 and three exp2, and
-// the majorant mu_max = max_density * max_m sum_x a_x[m] f_x, and 1/mu_max
-// (infinity when mu_max is 0: every flight then leaves the box).
-__device__ __forceinline__ void cross_sections(const drr_mc_params_t &p, const float (*s_a)[DRR_MAX_MATERIALS],
-                                               float E, float &fpe, float &fc, float &fr, float &mumax,
-                                               float &imumax) {
+// (E / 60 keV)^(-k) of the three processes from one log2 and three exp2.
+__device__ __forceinline__ void cross_sections(const drr_mc_params_t &p, float E, float &fpe, float &fc, float &fr) {
     const float l = __log2f(E * (1.0f / 60.0f));
     fpe = exp2f(-p.k_pe * l);
     fc = exp2f(-p.k_compton * l);
     fr = exp2f(-p.k_rayleigh * l);
-    float smax = 0.0f;
-    for (int m = 0; m < p.n_materials; m++) smax = fmaxf(smax, s_a[0][m] * fpe + s_a[1][m] * fc + s_a[2][m] * fr);
-    mumax = p.max_density * smax;
-    imumax = mumax > 0.0f ? 1.0f / mumax : INFINITY;
 }
 constexpr float E_MIN = 5.0f;  // keV: lower energies are absorbed locally
+constexpr int BRICK = 8;       // voxels per brick edge
+
+struct Bricks {
+    const float4 *maj;  // [nbz][nby][nbx]: (max rho a_pe, max rho a_c, max rho a_r, 0)
+    int nb[3];
+};
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int nx, int ny, int nz, drr_mc_params_t p, int64_t first,
-                                            int64_t count, uint64_t seed, float *img, float *img2,
-                                            unsigned long long *nsteps) {
+// One wave per brick: lane l reads voxel row (l & 7, l >> 3) of each of the
+// brick's 8 z-slices (8 contiguous records per row), then the wave reduces.
+__global__ __launch_bounds__(256) void k_mc_bricks(const uint2 *__restrict__ vox, int nx, int ny, int nz,
+                                                   drr_mc_params_t p, Bricks b, int64_t n_bricks) {
+    const int64_t bi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (bi >= n_bricks) return;  // whole waves leave together
+    const int bx = (int)(bi % b.nb[0]), by = (int)((bi / b.nb[0]) % b.nb[1]), bz = (int)(bi / ((int64_t)b.nb[0] * b.nb[1]));
+    const int i = bx * BRICK + (lane & 7), j = by * BRICK + (lane >> 3);
+    float mpe = 0.0f, mc = 0.0f, mr = 0.0f;
+    if (i < nx && j < ny) {
+        for (int dz = 0; dz < BRICK; dz++) {
+            const int k = bz * BRICK + dz;
+            if (k >= nz) break;
+            const uint2 rv = vox[((int64_t)k * ny + j) * nx + i];
+            const float rho = __uint_as_float(rv.x);
+            const int m = (int)rv.y;
+            mpe = fmaxf(mpe, rho * p.a_pe[m]);
+            mc = fmaxf(mc, rho * p.a_compton[m]);
+            mr = fmaxf(mr, rho * p.a_rayleigh[m]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mpe = fmaxf(mpe, __shfl_xor(mpe, o));
+        mc = fmaxf(mc, __shfl_xor(mc, o));
+        mr = fmaxf(mr, __shfl_xor(mr, o));
+    }
+    if (lane == 0) const_cast<float4 *>(b.maj)[bi] = make_float4(mpe, mc, mr, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int nx, int ny, int nz, drr_mc_params_t p,
+                                            Bricks bk, int64_t first, int64_t count, uint64_t seed, float *img,
+                                            float *img2, unsigned long long *nsteps, unsigned long long *nbsteps) {
     __shared__ float s_e[DRR_MAX_BINS], s_cdf[DRR_MAX_BINS];
     __shared__ float s_a[3][DRR_MAX_MATERIALS];
     for (int i = threadIdx.x; i < p.n_bins; i += blockDim.x) {
@@ -159,11 +201,11 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
     const int W = p.det_width, H = p.det_height;
     const int64_t npix = (int64_t)W * H;
     const float *A = p.ijk_from_world;
-    unsigned long long steps = 0;
+    unsigned long long steps = 0, bsteps = 0;
     Photon ph;
     ph.alive = false;
     Ranecu rng{1, 1};
-    float fpe = 0.f, fc = 0.f, fr = 0.f, mumax = 0.f, imumax = INFINITY;
+    float fpe = 0.f, fc = 0.f, fr = 0.f;
     float t_exit = 0.f;
     for (;;) {
         if (!ph.alive) {
@@ -208,14 +250,59 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
             if (ph.inside) {
                 for (int a = 0; a < 3; a++) ph.x[a] += tmin * ph.d[a];
                 t_exit = tmax - tmin;
-                cross_sections(p, s_a, ph.E, fpe, fc, fr, mumax, imumax);
+                cross_sections(p, ph.E, fpe, fc, fr);
             }
         }
         if (ph.inside) {
-            // one Woodcock flight
+            // one flight: optical depth tau spent brick by brick at each
+            // brick's majorant, up to a tentative collision or the box exit
             ++steps;
-            const float s = -__logf(ranecu(rng)) * imumax;
-            if (s >= t_exit) {
+            float tau = -__logf(ranecu(rng));
+            int bi[3], sg[3];
+            float tn[3], td[3];
+            for (int a = 0; a < 3; a++) {
+                const float qp = A[4 * a] * ph.x[0] + A[4 * a + 1] * ph.x[1] + A[4 * a + 2] * ph.x[2] + A[4 * a + 3] + 0.5f;
+                const float dq = A[4 * a] * ph.d[0] + A[4 * a + 1] * ph.d[1] + A[4 * a + 2] * ph.d[2];
+                bi[a] = min(max((int)floorf(qp * 0.125f), 0), bk.nb[a] - 1);
+                if (dq > 0.0f) {
+                    tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;
+                    td[a] = (float)BRICK / dq;
+                    sg[a] = 1;
+                } else if (dq < 0.0f) {
+                    tn[a] = ((float)(BRICK * bi[a]) - qp) / dq;
+                    td[a] = -(float)BRICK / dq;
+                    sg[a] = -1;
+                } else {
+                    tn[a] = INFINITY;
+                    td[a] = INFINITY;
+                    sg[a] = 0;
+                }
+                tn[a] = fmaxf(tn[a], 0.0f);
+            }
+            float t = 0.0f, mub = 0.0f, s = 0.0f;
+            bool coll = false;
+            for (;;) {
+                ++bsteps;
+                const float4 mj = bk.maj[((int64_t)bi[2] * bk.nb[1] + bi[1]) * bk.nb[0] + bi[0]];
+                mub = (mj.x * fpe + mj.y * fc) + mj.z * fr;
+                const int a = (tn[0] <= tn[1] && tn[0] <= tn[2]) ? 0 : (tn[1] <= tn[2] ? 1 : 2);
+                const float tnext = fminf(tn[a], t_exit);
+                const float seg = tnext - t;
+                if (mub > 0.0f) {
+                    if (tau < mub * seg) {
+                        s = t + tau / mub;
+                        coll = true;
+                        break;
+                    }
+                    tau -= mub * seg;
+                }
+                t = tnext;
+                if (tnext >= t_exit) break;
+                bi[a] += sg[a];
+                if (bi[a] < 0 || bi[a] >= bk.nb[a]) break;
+                tn[a] += td[a];
+            }
+            if (!coll) {
                 for (int a = 0; a < 3; a++) ph.x[a] += t_exit * ph.d[a];
                 ph.inside = false;  // left the box: fly to the detector below
             } else {
@@ -235,7 +322,7 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
                 const float mc = rho * s_a[1][m] * fc;
                 const float mr = rho * s_a[2][m] * fr;
                 const float mt = mpe + mc + mr;
-                if (ranecu(rng) * mumax < mt) {
+                if (ranecu(rng) * mub < mt) {
                     const float u = ranecu(rng) * mt;
                     float ct;
                     if (u < mpe) {
@@ -265,7 +352,7 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
                         }
                     }
                     t_exit = fmaxf(tmax, 0.0f);
-                    cross_sections(p, s_a, ph.E, fpe, fc, fr, mumax, imumax);
+                    cross_sections(p, ph.E, fpe, fc, fr);
                 }
             }
             continue;
@@ -295,11 +382,15 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
         for (int o = 32; o > 0; o >>= 1) steps += __shfl_xor(steps, o);
         if ((threadIdx.x & 63) == 0) atomicAdd(nsteps, steps);
     }
+    if (nbsteps) {
+        for (int o = 32; o > 0; o >>= 1) bsteps += __shfl_xor(bsteps, o);
+        if ((threadIdx.x & 63) == 0) atomicAdd(nbsteps, bsteps);
+    }
 }
 
 extern "C" int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_params_t *params,
                               int64_t first_history, int64_t n_histories, uint64_t seed, float *image,
-                              float *image2, int64_t *n_steps, void *stream) {
+                              float *image2, int64_t *n_steps, int64_t *n_brick_steps, void *stream) {
     if (!ctx || !vol || !params || !image) return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: NULL argument");
     if (vol->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: volume of another context");
     const drr_mc_params_t &p = *params;
@@ -321,8 +412,7 @@ extern "C" int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_
     drr_volume *v = const_cast<drr_volume *>(vol);
     int rc = drr_volume_ensure_linear(v);
     if (rc) return rc;
-    drr_mc_params_t pp = p;
-    if (!(pp.max_density > 0.0f) || pp.max_density < v->max_density) pp.max_density = v->max_density;
+    const drr_mc_params_t &pp = p;
     int cus = 0;
     DRR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     // Every lane takes histories h, h + stride, ...: a grid larger than what is
@@ -334,13 +424,29 @@ extern "C" int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_
     const int64_t cap = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
     if (blocks > cap) blocks = cap;
     hipStream_t st = (hipStream_t)stream;
+    // the brick majorants of these cross sections, in stream-ordered memory
+    Bricks bk{};
+    bk.nb[0] = (v->nx + BRICK - 1) / BRICK;
+    bk.nb[1] = (v->ny + BRICK - 1) / BRICK;
+    bk.nb[2] = (v->nz + BRICK - 1) / BRICK;
+    const int64_t n_bricks = (int64_t)bk.nb[0] * bk.nb[1] * bk.nb[2];
+    float4 *maj = nullptr;
+    hipError_t e = hipMallocAsync((void **)&maj, sizeof(float4) * n_bricks, st);
+    if (e != hipSuccess) {
+        if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
+        return drr_hip_error(e, "hipMallocAsync(brick majorants)");
+    }
+    bk.maj = maj;
     (void)hipEventRecord(ctx->ev0, st);
-    k_mc<<<(unsigned)blocks, 256, 0, st>>>(v->lin_vox, v->nx, v->ny, v->nz, pp, first_history,
+    k_mc_bricks<<<(unsigned)((n_bricks + 3) / 4), 256, 0, st>>>(v->lin_vox, v->nx, v->ny, v->nz, pp, bk, n_bricks);
+    k_mc<<<(unsigned)blocks, 256, 0, st>>>(v->lin_vox, v->nx, v->ny, v->nz, pp, bk, first_history,
                                          n_histories, seed, image, image2,
-                                         reinterpret_cast<unsigned long long *>(n_steps));
-    hipError_t e = hipGetLastError();
+                                         reinterpret_cast<unsigned long long *>(n_steps),
+                                         reinterpret_cast<unsigned long long *>(n_brick_steps));
+    e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;
+    (void)hipFreeAsync(maj, st);
     if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
     if (e != hipSuccess) return drr_hip_error(e, "drr_mc_scatter launch");
     return DRR_OK;

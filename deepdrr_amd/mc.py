@@ -120,7 +120,7 @@ class MonteCarlo:
         H, W = params.det_height, params.det_width
         self.img = torch.zeros((4, H, W), dtype=torch.float32, device=self.device)
         self.img2 = torch.zeros((4, H, W), dtype=torch.float32, device=self.device)
-        self.steps = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.steps = torch.zeros(2, dtype=torch.int64, device=self.device)  # flights, brick steps
         self.n = 0
 
     def run(self, n_histories: int, seed: int = 12345, first: int = 0, chunk: int = 1 << 24,
@@ -130,14 +130,22 @@ class MonteCarlo:
         while done < n_histories:
             k = min(chunk, n_histories - done)
             _abi.call("drr_mc_scatter", self.ctx, self.vol, C.byref(self.params), first + done, k,
-                      C.c_uint64(seed), _abi.ptr(self.img), _abi.ptr(self.img2), _abi.ptr(self.steps), s)
+                      C.c_uint64(seed), _abi.ptr(self.img), _abi.ptr(self.img2), _abi.ptr(self.steps),
+                      _abi.ptr(self.steps) + 8, s)
             done += k
         self.n += n_histories
 
     def results(self):
+        """Energy images [4, H, W] and sums of squares (float64) and the number
+        of Woodcock flights."""
         self.torch.cuda.synchronize(self.device)
         return (self.img.cpu().numpy().astype(np.float64), self.img2.cpu().numpy().astype(np.float64),
-                int(self.steps.item()))
+                int(self.steps[0].item()))
+
+    def brick_steps(self) -> int:
+        """Brick majorants crossed by the flights so far."""
+        self.torch.cuda.synchronize(self.device)
+        return int(self.steps[1].item())
 
     def last_kernel_ms(self) -> float:
         ms = C.c_float()

@@ -308,7 +308,7 @@ typedef struct {
     /* mu_x[m] (1/mm per g/cm^3) = a_x[m] * (E / 60 keV)^(-k_x) */
     float a_pe[DRR_MAX_MATERIALS], a_compton[DRR_MAX_MATERIALS], a_rayleigh[DRR_MAX_MATERIALS];
     float k_pe, k_compton, k_rayleigh;
-    float max_density;                /* majorant density (g/cm^3) */
+    float max_density;                /* unused (the majorants are per 8^3-voxel brick) */
     float source_world[3];            /* point source */
     float world_from_ijk[12];         /* 3x4 row-major */
     float ijk_from_world[12];         /* 3x4 row-major */
@@ -330,11 +330,15 @@ typedef struct {
  * multiple] (the column order of MC-GPU's output, test_mcgpu.py:198-235).
  * image2 (optional) receives the per-pixel sums of squared energies (for
  * variance estimates); n_steps (optional, device int64) receives the number
- * of Woodcock flight steps.  All three accumulate.  The volume is sampled at
- * voxel centres (nearest voxel), IJK box [-0.5, n-0.5]. */
+ * of Woodcock flights (each ends in a tentative collision or leaves the
+ * volume) and n_brick_steps (optional, device int64) the number of brick
+ * majorants they crossed.  All four accumulate.  Delta tracking uses a local
+ * majorant per 8x8x8-voxel brick (built per call in stream-ordered memory).
+ * The volume is sampled at voxel centres (nearest voxel), IJK box
+ * [-0.5, n-0.5]. */
 int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_params_t *params,
                    int64_t first_history, int64_t n_histories, uint64_t seed,
-                   float *image, float *image2, int64_t *n_steps, void *stream);
+                   float *image, float *image2, int64_t *n_steps, int64_t *n_brick_steps, void *stream);
 
 /* ---- timing helper ---------------------------------------------------------- */
 

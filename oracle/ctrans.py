@@ -174,7 +174,8 @@ def _mc_lib():
 
         l.ora_mc.restype = C.c_long
         l.ora_mc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(MCParams),
-                             C.c_int64, C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
+                             C.c_int64, C.c_int64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int,
+                             C.POINTER(C.c_long)]
         l.ora_sample_compton.restype = None
         l.ora_sample_compton.argtypes = [C.c_float, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p]
         l.ora_sample_rayleigh.restype = None
@@ -192,17 +193,22 @@ def _mc_lib():
     return l
 
 
-def mc(density_kji, labels_kji, params, n_histories, seed=12345, first=0, n_threads=None):
-    """CPU Monte Carlo: returns img [4,H,W], img2 [4,H,W] (float64) and the step count."""
+def mc(density_kji, labels_kji, params, n_histories, seed=12345, first=0, n_threads=None, brick_steps=None):
+    """CPU Monte Carlo: returns img [4,H,W], img2 [4,H,W] (float64) and the
+    number of Woodcock flights; ``brick_steps`` (a list) receives the number of
+    brick majorants crossed."""
     d = np.ascontiguousarray(density_kji, np.float32)
     l = np.ascontiguousarray(labels_kji, np.uint8)
     nz, ny, nx = d.shape
     H, W = params.det_height, params.det_width
     img = np.zeros((4, H, W), np.float64)
     img2 = np.zeros((4, H, W), np.float64)
+    bs = C.c_long(0)
     steps = _mc_lib().ora_mc(d.ctypes.data, l.ctypes.data, nx, ny, nz, C.byref(params), first,
                              n_histories, C.c_uint64(seed), img.ctypes.data, img2.ctypes.data,
-                             int(n_threads or default_threads()))
+                             int(n_threads or default_threads()), C.byref(bs))
+    if brick_steps is not None:
+        brick_steps.append(int(bs.value))
     return img, img2, int(steps)
 
 

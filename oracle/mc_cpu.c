@@ -5,7 +5,9 @@
  * the product never links it).  deepdrr has no scatter kernel any more
  * (projector.py:527-531); this restates the MC-GPU method the engine
  * implements: point source, spectrum sampling by the cumulative pdf, emission
- * uniform in a cone, Woodcock tracking against a majorant, photoelectric /
+ * uniform in a cone, Woodcock tracking against a local majorant per
+ * 8x8x8-voxel brick (component-wise maxima of density x the three process
+ * coefficients; a flight spends its optical depth brick by brick), photoelectric /
  * Compton (Klein-Nishina, Kahn's method) / Rayleigh (Thomson) interactions,
  * scoring on the detector as [primary, single Compton, single Rayleigh,
  * multiple].  Random numbers: RANECU (L'Ecuyer 1988), one stream per history
@@ -107,14 +109,86 @@ static float rayleigh(ranecu_t *rng) {
 #define TWO_PI 6.283185307179586f
 #define E_MIN 5.0f
 
-static float majorant(const drr_mc_params_t *p, float E, float *fpe, float *fc, float *fr) {
+static void cross_sections(const drr_mc_params_t *p, float E, float *fpe, float *fc, float *fr) {
     *fpe = powf(E / 60.0f, -p->k_pe);
     *fc = powf(E / 60.0f, -p->k_compton);
     *fr = powf(E / 60.0f, -p->k_rayleigh);
-    float smax = 0.0f;
-    for (int m = 0; m < p->n_materials; m++)
-        smax = fmaxf(smax, p->a_pe[m] * *fpe + p->a_compton[m] * *fc + p->a_rayleigh[m] * *fr);
-    return p->max_density * smax;
+}
+
+#define BRICK 8
+
+typedef struct {
+    int nb[3];
+    float *maj; /* [nbz][nby][nbx][3]: max rho a_pe, max rho a_c, max rho a_r */
+} bricks_t;
+
+static void build_bricks(const float *dens, const uint8_t *lab, const int *n, const drr_mc_params_t *p,
+                         bricks_t *b) {
+    for (int a = 0; a < 3; a++) b->nb[a] = (n[a] + BRICK - 1) / BRICK;
+    size_t nbr = (size_t)b->nb[0] * b->nb[1] * b->nb[2];
+    b->maj = (float *)calloc(3 * nbr, sizeof(float));
+    for (int k = 0; k < n[2]; k++)
+        for (int j = 0; j < n[1]; j++)
+            for (int i = 0; i < n[0]; i++) {
+                size_t vi = ((size_t)k * n[1] + j) * n[0] + i;
+                size_t bi = (((size_t)(k / BRICK) * b->nb[1] + j / BRICK) * b->nb[0] + i / BRICK) * 3;
+                float rho = dens[vi];
+                int m = lab[vi];
+                b->maj[bi] = fmaxf(b->maj[bi], rho * p->a_pe[m]);
+                b->maj[bi + 1] = fmaxf(b->maj[bi + 1], rho * p->a_compton[m]);
+                b->maj[bi + 2] = fmaxf(b->maj[bi + 2], rho * p->a_rayleigh[m]);
+            }
+}
+
+/* One flight from x along d with optical depth tau: walks the bricks
+ * (Amanatides-Woo) and returns the distance of the tentative collision and its
+ * brick majorant in *mub, or -1 if the photon leaves the box (t_exit) first.
+ * *bsteps counts the bricks crossed. */
+static float flight(const bricks_t *b, const float *A, const float *x, const float *d, float tau, float t_exit,
+                    float fpe, float fc, float fr, float *mub_out, long *bsteps) {
+    int bi[3], sg[3];
+    float tn[3], td[3];
+    for (int a = 0; a < 3; a++) {
+        float qp = A[4 * a] * x[0] + A[4 * a + 1] * x[1] + A[4 * a + 2] * x[2] + A[4 * a + 3] + 0.5f;
+        float dq = A[4 * a] * d[0] + A[4 * a + 1] * d[1] + A[4 * a + 2] * d[2];
+        int c = (int)floorf(qp * 0.125f);
+        bi[a] = c < 0 ? 0 : (c > b->nb[a] - 1 ? b->nb[a] - 1 : c);
+        if (dq > 0.0f) {
+            tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;
+            td[a] = (float)BRICK / dq;
+            sg[a] = 1;
+        } else if (dq < 0.0f) {
+            tn[a] = ((float)(BRICK * bi[a]) - qp) / dq;
+            td[a] = -(float)BRICK / dq;
+            sg[a] = -1;
+        } else {
+            tn[a] = INFINITY;
+            td[a] = INFINITY;
+            sg[a] = 0;
+        }
+        tn[a] = fmaxf(tn[a], 0.0f);
+    }
+    float t = 0.0f;
+    for (;;) {
+        ++*bsteps;
+        const float *mj = &b->maj[(((size_t)bi[2] * b->nb[1] + bi[1]) * b->nb[0] + bi[0]) * 3];
+        float mub = (mj[0] * fpe + mj[1] * fc) + mj[2] * fr;
+        int a = (tn[0] <= tn[1] && tn[0] <= tn[2]) ? 0 : (tn[1] <= tn[2] ? 1 : 2);
+        float tnext = fminf(tn[a], t_exit);
+        float seg = tnext - t;
+        if (mub > 0.0f) {
+            if (tau < mub * seg) {
+                *mub_out = mub;
+                return t + tau / mub;
+            }
+            tau -= mub * seg;
+        }
+        t = tnext;
+        if (tnext >= t_exit) return -1.0f;
+        bi[a] += sg[a];
+        if (bi[a] < 0 || bi[a] >= b->nb[a]) return -1.0f;
+        tn[a] += td[a];
+    }
 }
 
 static float qcoord(const float *A, int a, const float *x) {
@@ -133,8 +207,8 @@ enum { T_EMITTED, T_SCORED, T_PHOTO, T_COMPTON, T_CUTOFF, T_ESCAPED, T_N };
  * *steps counts Woodcock flights; tally (may be NULL) accumulates the energy
  * balance of the history. */
 static int history(const float *dens, const uint8_t *lab, const int *n, const drr_mc_params_t *p,
-                   uint64_t seed, uint64_t h, float *E_out, int *iu_out, int *iv_out, long *steps,
-                   double *tally) {
+                   const bricks_t *bk, uint64_t seed, uint64_t h, float *E_out, int *iu_out, int *iv_out,
+                   long *steps, long *bsteps, double *tally) {
     ranecu_t rng = seed_history(seed, h);
     float u = ranecu(&rng);
     int lo = 0, hi = p->n_bins - 1;
@@ -166,11 +240,12 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
         for (int a = 0; a < 3; a++) x[a] += tmin * d[a];
         float t_exit = tmax - tmin;
         float fpe, fc, fr;
-        float mumax = majorant(p, E, &fpe, &fc, &fr);
+        cross_sections(p, E, &fpe, &fc, &fr);
         for (;;) {
             ++*steps;
-            float s = (mumax > 0.0f) ? -logf(ranecu(&rng)) / mumax : INFINITY;
-            if (s >= t_exit) {
+            float mub = 0.0f;
+            float s = flight(bk, A, x, d, -logf(ranecu(&rng)), t_exit, fpe, fc, fr, &mub, bsteps);
+            if (s < 0.0f) {
                 for (int a = 0; a < 3; a++) x[a] += t_exit * d[a];
                 break;
             }
@@ -188,7 +263,7 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
             float mc = rho * p->a_compton[m] * fc;
             float mr = rho * p->a_rayleigh[m] * fr;
             float mt = mpe + mc + mr;
-            if (ranecu(&rng) * mumax < mt) {
+            if (ranecu(&rng) * mub < mt) {
                 float uu = ranecu(&rng) * mt;
                 if (uu < mpe) {
                     if (tally) tally[T_PHOTO] += E;
@@ -214,7 +289,7 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
                     if (dq != 0.0f) tm = fminf(tm, fmaxf((-0.5f - q0) / dq, ((float)n[a] - 0.5f - q0) / dq));
                 }
                 t_exit = fmaxf(tm, 0.0f);
-                mumax = majorant(p, E, &fpe, &fc, &fr);
+                cross_sections(p, E, &fpe, &fc, &fr);
             }
         }
     }
@@ -250,22 +325,27 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
 }
 
 /* dens/lab [nz][ny][nx]; img/img2 double [4][H][W] (accumulated);
- * returns the number of Woodcock flights. */
+ * returns the number of Woodcock flights; *brick_steps (may be NULL)
+ * receives the number of brick majorants they crossed. */
 long ora_mc(const float *dens, const uint8_t *lab, int nx, int ny, int nz, const drr_mc_params_t *p,
-            int64_t first, int64_t count, uint64_t seed, double *img, double *img2, int n_threads) {
+            int64_t first, int64_t count, uint64_t seed, double *img, double *img2, int n_threads,
+            long *brick_steps) {
     const int n[3] = {nx, ny, nz};
     const long npix = (long)p->det_width * p->det_height;
-    long total_steps = 0;
-#pragma omp parallel num_threads(n_threads) reduction(+ : total_steps)
+    bricks_t bk;
+    build_bricks(dens, lab, n, p, &bk);
+    long total_steps = 0, total_bsteps = 0;
+#pragma omp parallel num_threads(n_threads) reduction(+ : total_steps, total_bsteps)
     {
         double *li = (double *)calloc(4 * npix, sizeof(double));
         double *l2 = (double *)calloc(4 * npix, sizeof(double));
-        long steps = 0;
+        long steps = 0, bsteps = 0;
 #pragma omp for schedule(dynamic, 256)
         for (int64_t i = 0; i < count; i++) {
             float E;
             int iu, iv;
-            int cls = history(dens, lab, n, p, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, NULL);
+            int cls = history(dens, lab, n, p, &bk, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, &bsteps,
+                              NULL);
             if (cls >= 0) {
                 long o = cls * npix + (long)iv * p->det_width + iu;
                 li[o] += E;
@@ -282,7 +362,10 @@ long ora_mc(const float *dens, const uint8_t *lab, int nx, int ny, int nz, const
         free(li);
         free(l2);
         total_steps += steps;
+        total_bsteps += bsteps;
     }
+    free(bk.maj);
+    if (brick_steps) *brick_steps = total_bsteps;
     return total_steps;
 }
 
@@ -310,15 +393,18 @@ void ora_sample_rayleigh(int n, uint64_t seed, float *cos_out) {
 long ora_mc_tally(const float *dens, const uint8_t *lab, int nx, int ny, int nz, const drr_mc_params_t *p,
                   int64_t first, int64_t count, uint64_t seed, double *tally, long *counts) {
     const int n[3] = {nx, ny, nz};
-    long steps = 0;
+    long steps = 0, bsteps = 0;
+    bricks_t bk;
+    build_bricks(dens, lab, n, p, &bk);
     for (int k = 0; k < T_N; k++) tally[k] = 0.0;
     for (int k = 0; k < 4; k++) counts[k] = 0;
     for (int64_t i = 0; i < count; i++) {
         float E;
         int iu, iv;
-        int cls = history(dens, lab, n, p, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, tally);
+        int cls = history(dens, lab, n, p, &bk, seed, (uint64_t)(first + i), &E, &iu, &iv, &steps, &bsteps, tally);
         if (cls >= 0) counts[cls]++;
     }
+    free(bk.maj);
     return steps;
 }
 

@@ -55,42 +55,79 @@ def _slab_setup(E_keV=60.0, det=16, n=20, rho=1.0):
     return vol, proj, spectrum
 
 
+def _expected_primary(p, n_hist, E, boxes):
+    """E * N * (Omega_pix / Omega_cone) * exp(-sum_b mu_b L_b): boxes are
+    (lo, hi, mu) in IJK with later boxes replacing earlier ones where they
+    overlap (each given as its own mu minus the enclosing box's)."""
+    src = np.array(p.source_world[:])
+    o = np.array(p.detector_origin[:])
+    du = np.array(p.detector_u[:])
+    dv = np.array(p.detector_v[:])
+    Wi = np.array(p.ijk_from_world[:]).reshape(3, 4)
+    H, W = p.det_height, p.det_width
+    expect = np.zeros((H, W))
+    frac = mc.pixel_solid_angles(p) / mc.cone_solid_angle(p)
+    for iv in range(H):
+        for iu in range(W):
+            tgt = o + (iu + 0.5) * du + (iv + 0.5) * dv
+            d = (tgt - src) / np.linalg.norm(tgt - src)
+            q0 = Wi[:, :3] @ src + Wi[:, 3]
+            dq = Wi[:, :3] @ d
+            tau = 0.0
+            for lo, hi, mu in boxes:
+                with np.errstate(divide="ignore"):
+                    t0 = (lo - q0) / dq
+                    t1 = (hi - q0) / dq
+                tau += mu * max(0.0, np.min(np.maximum(t0, t1)) - np.max(np.minimum(t0, t1)))
+            expect[iv, iu] = E * n_hist * frac[iv, iu] * math.exp(-tau)
+    return expect
+
+
+def _z(prim, expect, E):
+    sel = expect > 0
+    return (prim[sel] - expect[sel]) / (E * np.sqrt(expect[sel] / E))
+
+
 def test_woodcock_primary_matches_beer_lambert():
     vol, proj, spectrum = _slab_setup()
     d_kji = np.ascontiguousarray(np.moveaxis(vol.data, [0, 1, 2], [2, 1, 0]))
     l_kji = np.zeros_like(d_kji, dtype=np.uint8)
-    mats = ["soft tissue"]
-    for maj in (1.0, 3.0):
-        p = mc.make_params(mats, spectrum, proj, 400.0, vol.world_from_IJK.data, max_density=maj)
-        n_hist = 400000
-        img, img2, steps = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8)
-        prim = img[0]
-        # expected primary energy per pixel: E * N * (Omega_pix / Omega_cone) * exp(-mu L)
-        mu = sum(mc.SYNTHETIC_MU60["soft tissue"]) * 0.1 * 1.0  # 1/mm at 60 keV, rho = 1
-        src = np.array(p.source_world[:])
-        o = np.array(p.detector_origin[:])
-        du = np.array(p.detector_u[:])
-        dv = np.array(p.detector_v[:])
-        Wi = np.array(p.ijk_from_world[:]).reshape(3, 4)
-        H, W = p.det_height, p.det_width
-        expect = np.zeros((H, W))
-        frac = mc.pixel_solid_angles(p) / mc.cone_solid_angle(p)
-        for iv in range(H):
-            for iu in range(W):
-                tgt = o + (iu + 0.5) * du + (iv + 0.5) * dv
-                d = (tgt - src) / np.linalg.norm(tgt - src)
-                q0 = Wi[:, :3] @ src + Wi[:, 3]
-                dq = Wi[:, :3] @ d
-                t0 = (-0.5 - q0) / dq
-                t1 = (20 - 0.5 - q0) / dq
-                L = max(0.0, np.min(np.maximum(t0, t1)) - np.max(np.minimum(t0, t1)))
-                expect[iv, iu] = 60.0 * n_hist * frac[iv, iu] * math.exp(-mu * L)
-        sel = expect > 0
-        z = (prim[sel] - expect[sel]) / (60.0 * np.sqrt(expect[sel] / 60.0))
-        assert np.mean(np.abs(z) < 4) > 0.98, (maj, z)
-        assert abs(np.mean(z)) < 0.5
-        # the 3x majorant adds virtual collisions (each entering photon also takes
-        # one exit flight, so with mu L ~ 0.4 the step count grows ~1.6x)
-        if maj == 3.0:
-            assert steps > 1.3 * steps_exact
-        steps_exact = steps
+    p = mc.make_params(["soft tissue"], spectrum, proj, 400.0, vol.world_from_IJK.data)
+    n_hist = 400000
+    img, img2, steps = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8)
+    mu = sum(mc.SYNTHETIC_MU60["soft tissue"]) * 0.1 * 1.0  # 1/mm at 60 keV, rho = 1
+    z = _z(img[0], _expected_primary(p, n_hist, 60.0, [(-0.5, 19.5, mu)]), 60.0)
+    assert np.mean(np.abs(z) < 4) > 0.98, z
+    assert abs(np.mean(z)) < 0.5
+
+
+def test_local_majorants_with_a_dense_insert():
+    """A 4^3-voxel titanium insert (4.5 g/cm^3) in a soft-tissue cube: the
+    brick majorants keep the primary unbiased across bricks of very different
+    majorants (Beer-Lambert through both materials), and the flights grow far
+    less than the titanium/tissue attenuation ratio a global majorant would
+    multiply them by."""
+    vol, proj, spectrum = _slab_setup()
+    d_kji = np.ascontiguousarray(np.moveaxis(vol.data, [0, 1, 2], [2, 1, 0])).copy()
+    l_kji = np.zeros_like(d_kji, dtype=np.uint8)
+    mats = ["soft tissue", "titanium"]
+    p = mc.make_params(mats, spectrum, proj, 400.0, vol.world_from_IJK.data)
+    n_hist = 400000
+    _, _, steps_uniform = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8)
+    d_kji[8:12, 8:12, 8:12] = 4.5
+    l_kji[8:12, 8:12, 8:12] = 1
+    bsteps = []
+    img, img2, steps = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8, brick_steps=bsteps)
+    mu_st = sum(mc.SYNTHETIC_MU60["soft tissue"]) * 0.1
+    mu_ti = sum(mc.SYNTHETIC_MU60["titanium"]) * 0.1 * 4.5
+    assert mu_ti > 10 * mu_st
+    expect = _expected_primary(p, n_hist, 60.0, [(-0.5, 19.5, mu_st), (7.5, 11.5, mu_ti - mu_st)])
+    z = _z(img[0], expect, 60.0)
+    assert np.mean(np.abs(z) < 4) > 0.98, z
+    assert abs(np.mean(z)) < 0.5
+    # the insert's rays really are darker
+    assert expect.min() < 0.5 * expect.max()
+    # the insert's 8^3 brick carries the titanium majorant; a global majorant
+    # would scale every flight count by mu_ti / mu_st
+    assert steps < 0.2 * steps_uniform * (mu_ti / mu_st), (steps, steps_uniform, mu_ti / mu_st)
+    assert bsteps[0] > steps

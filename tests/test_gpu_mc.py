@@ -111,7 +111,7 @@ def test_mc_parameter_errors(gpu):
         img = torch.zeros((4, 8, 8), device=gpu)
 
         def run(p, n=1000):
-            return _abi.lib().drr_mc_scatter(ctx, h, C.byref(p), 0, n, C.c_uint64(1), _abi.ptr(img), None, None,
+            return _abi.lib().drr_mc_scatter(ctx, h, C.byref(p), 0, n, C.c_uint64(1), _abi.ptr(img), None, None, None,
                                              _abi.stream_ptr())
 
         def params(**kw):
