@@ -4,7 +4,8 @@ persistent launch replayed from a hipGraph.
 This is the batched form of deepdrr's per-view loop
 (``deepdrr/projector/projector.py:679-707``): the same images, computed by one
 graph launch with the neglog epilogue on the device and one device-to-host
-copy at the end.  Every job views exactly one volume.
+copy at the end.  A view over n volumes uses the plan's volumes
+``volume_index .. volume_index + n - 1``.
 """
 from __future__ import annotations
 
@@ -23,8 +24,8 @@ class BatchPlan:
     """A captured batch over device-resident volumes.
 
     ``ctx``: a drr_ctx handle; ``vol_handles``: drr_volume handles; ``views``:
-    ctypes ``View`` objects (one volume each) and ``volume_indices`` into
-    ``vol_handles``.  Outputs live in device tensors ``intensity``,
+    ctypes ``View`` objects and ``volume_indices`` into ``vol_handles`` (the
+    first of a view's ``n_volumes`` consecutive handles).  Outputs live in device tensors ``intensity``,
     ``photon_prob`` and (with ``neglog``) ``images``, each image at
     ``offsets[i]`` with shape ``shapes[i]``.
 
@@ -61,8 +62,8 @@ class BatchPlan:
         total = int(np.sum(sizes))
         jobs = (Job * n)()
         for i, (v, vi) in enumerate(zip(views, volume_indices)):
-            if v.n_volumes != 1:
-                raise ValueError("each batched view must contain exactly one volume")
+            if not 0 <= int(vi) <= len(vol_handles) - v.n_volumes:
+                raise ValueError(f"view {i}: volume index {vi} with {v.n_volumes} volumes is out of range")
             jobs[i].view = v
             jobs[i].volume_index = int(vi)
             jobs[i].out_offset = int(self.offsets[i])
@@ -129,17 +130,20 @@ def plan_for_projector(projector, projections: Sequence[geo.CameraProjection],
 
     if not projector.initialized:
         raise RuntimeError("Projector has not been initialized.")
-    if volume_indices is None:
-        if len(projector.volumes) != 1:
-            raise ValueError("volume_indices is required when the projector holds several volumes")
-        volume_indices = [0] * len(projections)
     if not hasattr(projector, "max_ray_length"):
         projector.max_ray_length = projector.source_to_detector_distance * 4
     views = []
     sdd = projector.source_to_detector_distance
-    for proj, vi in zip(projections, volume_indices):
-        views.append(build_view([projector.volumes[vi]], [projector.priorities[vi]], proj,
-                                projector.max_ray_length, sdd if sdd > 0 else 1.0))
+    if volume_indices is None:
+        # every view over all the projector's volumes, as project() renders it
+        volume_indices = [0] * len(projections)
+        for proj in projections:
+            views.append(build_view(projector.volumes, projector.priorities, proj, projector.max_ray_length,
+                                    sdd if sdd > 0 else 1.0))
+    else:
+        for proj, vi in zip(projections, volume_indices):
+            views.append(build_view([projector.volumes[vi]], [projector.priorities[vi]], proj,
+                                    projector.max_ray_length, sdd if sdd > 0 else 1.0))
     key = None
     if tuning == "auto":
         from . import autotune

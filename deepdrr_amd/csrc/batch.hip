@@ -30,12 +30,19 @@ void drr_jobs_free(JobTable &t) {
     (void)hipFree(t.d_items);
     (void)hipFree(t.d_seg);
     (void)hipFree(t.d_counters);
+    (void)hipFree(t.d_mjobs);
+    (void)hipFree(t.d_mitems);
     t.d_jobs = nullptr;
     t.d_vols = nullptr;
     t.d_items = nullptr;
     t.d_seg = nullptr;
     t.d_counters = nullptr;
+    t.d_mjobs = nullptr;
+    t.d_mitems = nullptr;
 }
+
+static int jobs_build_impl(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                           int n_jobs, const drr_batch_tuning_t &tune, bool allow_multi, JobTable &t);
 
 int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                    int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
@@ -43,7 +50,7 @@ int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, cons
         return drr_set_error(DRR_ERR_INVALID, "jobs: tile must be 8x8 or 16x4");
     if (tune.waves_per_block < 1 || tune.waves_per_block > 4 || tune.blocks_per_cu < 1 || tune.blocks_per_cu > 32)
         return drr_set_error(DRR_ERR_INVALID, "jobs: waves_per_block in [1,4], blocks_per_cu in [1,32]");
-    return drr_jobs_build_tiles(ctx, vols, n_vols, jobs, n_jobs, tune, t);
+    return jobs_build_impl(ctx, vols, n_vols, jobs, n_jobs, tune, true, t);
 }
 
 namespace {
@@ -54,13 +61,18 @@ struct JobHost {
     std::vector<VolDev> vols;
     std::vector<int2> items;
     int seg[9];
+    std::vector<JobMultiDev> mjobs;
+    std::vector<int2> mitems;
 };
 
 // Validate the jobs and fill `h` and the host fields of `t` (n_jobs, n_vols,
-// imgs, max_npix, n_items, grid, block).  A plan with more than `max_items`
-// ray tiles is refused before its tile list is built.
+// imgs, max_npix, n_items, grid, block, the multi-volume groups).  A plan
+// with more than `max_items` ray tiles is refused before its tile list is
+// built.  A job over V > 1 volumes (allow_multi) reads vols[volume_index ..
+// volume_index + V - 1] as its view.vol[0 .. V-1].
 int jobs_prepare(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs, int n_jobs,
-                 const drr_batch_tuning_t &tune, int64_t max_items, const char *who, JobTable &t, JobHost &h) {
+                 const drr_batch_tuning_t &tune, int64_t max_items, const char *who, bool allow_multi, JobTable &t,
+                 JobHost &h) {
     if (n_vols < 1 || n_jobs < 1 || n_jobs > 65535)
         return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": need >= 1 volume and 1..65535 jobs");
     if (tune.tile_w < 1 || tune.tile_h < 1 || tune.waves_per_block < 1 || tune.blocks_per_cu < 1)
@@ -74,28 +86,46 @@ int jobs_prepare(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const 
     t.imgs.assign(n_jobs, ImgSeg{0, 0});
     std::vector<int> order(n_jobs);
     t.max_npix = 0;
-    int64_t n_items = 0;
+    int64_t n_items = 0, n_mitems = 0;
+    std::vector<int> mjob_of(n_jobs, -1);
+    h.mjobs.clear();
+    int n_single = 0;
     for (int j = 0; j < n_jobs; j++) {
         const drr_job_t &J = jobs[j];
-        if (J.view.n_volumes != 1)
+        const int V = J.view.n_volumes;
+        if (V < 1 || V > DRR_MAX_VOLUMES)
+            return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": view.n_volumes must be in [1, 4]");
+        if (V != 1 && !allow_multi)
             return drr_set_error(DRR_ERR_UNSUPPORTED, std::string(who) + ": each job must view exactly one volume");
-        if (J.volume_index < 0 || J.volume_index >= n_vols)
+        if (J.volume_index < 0 || J.volume_index + V > n_vols)
             return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": volume_index out of range");
         if (J.view.width < 1 || J.view.height < 1 || J.view.width > 65536 || J.view.height > 65536 || J.out_offset < 0)
             return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": bad image size or offset");
         if (drr_check_view_finite(&J.view) != DRR_OK) return DRR_ERR_INVALID;
+        t.imgs[j] = ImgSeg{J.out_offset, (int64_t)J.view.width * J.view.height};
+        t.max_npix = std::max<int64_t>(t.max_npix, t.imgs[j].npix);
+        if (V > 1) {
+            JobMultiDev md{};
+            view_to_dev(J.view, vols + J.volume_index, md.vd);
+            md.tiles_x = (J.view.width + 7) / 8;
+            md.out_offset = J.out_offset;
+            mjob_of[j] = (int)h.mjobs.size();
+            h.mjobs.push_back(md);
+            n_mitems += (int64_t)md.tiles_x * ((J.view.height + 7) / 8);
+            continue;
+        }
         view_to_geo1(J.view, h.jobs[j].g);
         h.jobs[j].vol = J.volume_index;
         h.jobs[j].tiles_x = (J.view.width + tune.tile_w - 1) / tune.tile_w;
         h.jobs[j].out_offset = J.out_offset;
-        t.imgs[j] = ImgSeg{J.out_offset, (int64_t)J.view.width * J.view.height};
-        t.max_npix = std::max<int64_t>(t.max_npix, t.imgs[j].npix);
         n_items += (int64_t)h.jobs[j].tiles_x * ((J.view.height + tune.tile_h - 1) / tune.tile_h);
-        order[j] = j;
+        order[n_single++] = j;
     }
-    if (n_items >= ((int64_t)1 << 31)) return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": too many ray tiles");
-    if (max_items >= 0 && n_items > max_items)
-        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": the batch has " + std::to_string(n_items) +
+    order.resize(n_single);
+    if (n_items + n_mitems >= ((int64_t)1 << 31))
+        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": too many ray tiles");
+    if (max_items >= 0 && n_items + n_mitems > max_items)
+        return drr_set_error(DRR_ERR_INVALID, std::string(who) + ": the batch has " + std::to_string(n_items + n_mitems) +
                                                   " ray tiles, more than the " + std::to_string(max_items) +
                                                   " its scratch holds");
     // group tiles by volume (L2 / MALL reuse across poses), then by job
@@ -108,6 +138,20 @@ int jobs_prepare(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const 
         for (int k = 0; k < n; k++) h.items.push_back(make_int2(j, k));
     }
     t.n_items = n_items;
+    // multi-volume tiles, grouped by volume count (one kernel instance each)
+    h.mitems.clear();
+    h.mitems.reserve((size_t)n_mitems);
+    for (int V = 0; V <= DRR_MAX_VOLUMES; V++) {
+        t.m_first[V] = (int64_t)h.mitems.size();
+        for (int j = 0; j < n_jobs; j++) {
+            if (mjob_of[j] < 0 || jobs[j].view.n_volumes != V) continue;
+            const JobMultiDev &md = h.mjobs[mjob_of[j]];
+            const int n = md.tiles_x * ((jobs[j].view.height + 7) / 8);
+            for (int k = 0; k < n; k++) h.mitems.push_back(make_int2(mjob_of[j], k));
+        }
+        t.m_count[V] = (int64_t)h.mitems.size() - t.m_first[V];
+    }
+    t.n_mjobs = (int)h.mjobs.size();
     for (int q = 0; q <= 8; q++) h.seg[q] = (int)(t.n_items * q / 8);
     h.vols.resize(n_vols);
     for (int v = 0; v < n_vols; v++) h.vols[v] = vol_to_dev(vols[v]);
@@ -122,8 +166,13 @@ int jobs_prepare(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const 
 
 int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                          int n_jobs, const drr_batch_tuning_t &tune, JobTable &t) {
+    return jobs_build_impl(ctx, vols, n_vols, jobs, n_jobs, tune, false, t);
+}
+
+static int jobs_build_impl(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
+                           int n_jobs, const drr_batch_tuning_t &tune, bool allow_multi, JobTable &t) {
     JobHost h;
-    int rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, -1, "jobs", t, h);
+    int rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, -1, "jobs", allow_multi, t, h);
     if (rc != DRR_OK) return rc;
     hipError_t e;
 #define JCHK(x)                                             \
@@ -136,13 +185,20 @@ int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols
     } while (0)
     JCHK(hipMalloc(&t.d_jobs, sizeof(JobDev) * n_jobs));
     JCHK(hipMalloc(&t.d_vols, sizeof(VolDev) * n_vols));
-    JCHK(hipMalloc(&t.d_items, sizeof(int2) * h.items.size()));
+    JCHK(hipMalloc(&t.d_items, sizeof(int2) * std::max<size_t>(1, h.items.size())));
     JCHK(hipMalloc(&t.d_seg, sizeof(int) * 9));
     JCHK(hipMalloc(&t.d_counters, sizeof(int) * 8));
     JCHK(hipMemcpy(t.d_jobs, h.jobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
     JCHK(hipMemcpy(t.d_vols, h.vols.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice));
-    JCHK(hipMemcpy(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice));
+    if (!h.items.empty())
+        JCHK(hipMemcpy(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice));
     JCHK(hipMemcpy(t.d_seg, h.seg, sizeof(int) * 9, hipMemcpyHostToDevice));
+    if (!h.mjobs.empty()) {
+        JCHK(hipMalloc(&t.d_mjobs, sizeof(JobMultiDev) * h.mjobs.size()));
+        JCHK(hipMalloc(&t.d_mitems, sizeof(int2) * h.mitems.size()));
+        JCHK(hipMemcpy(t.d_mjobs, h.mjobs.data(), sizeof(JobMultiDev) * h.mjobs.size(), hipMemcpyHostToDevice));
+        JCHK(hipMemcpy(t.d_mitems, h.mitems.data(), sizeof(int2) * h.mitems.size(), hipMemcpyHostToDevice));
+    }
 #undef JCHK
     return DRR_OK;
 }
@@ -214,6 +270,62 @@ static void launch_march(const JobTable &t, int M, bool atten, int tw, hipStream
     }
 }
 
+// Jobs over 2..4 volumes: one 8x8-pixel tile per wave, the literal
+// multi-volume march (march_multi, as drr_project runs such a view).
+template <int V, int M>
+__global__ __launch_bounds__(256) void k_batch_multi(const JobMultiDev *__restrict__ jobs, const int2 *__restrict__ items,
+                                                     int64_t n_items, SpectrumDev sp, float step, int atten, int air,
+                                                     float *__restrict__ I, float *__restrict__ P,
+                                                     float *__restrict__ AD) {
+    const int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= n_items) return;  // whole waves leave together
+    const int lane = threadIdx.x & 63;
+    const int2 item = items[it];
+    const int j = __builtin_amdgcn_readfirstlane(item.x);
+    const int tile = __builtin_amdgcn_readfirstlane(item.y);
+    const JobMultiDev &J = jobs[j];
+    int udx, vdx;
+    lane_pixel(tile % J.tiles_x, tile / J.tiles_x, lane, udx, vdx);
+    if (udx >= J.vd.width || vdx >= J.vd.height) return;
+    float ad[M];
+    march_multi<V, M>(J.vd, udx, vdx, step, atten, air, ad);
+    float inten, pp;
+    spectral<M>(sp, ad, inten, pp);
+    const int64_t npix = (int64_t)J.vd.width * J.vd.height;
+    const int64_t pix = (int64_t)vdx * J.vd.width + udx;
+    I[J.out_offset + pix] = inten;
+    if (P) P[J.out_offset + pix] = pp;
+    if (AD) {
+#pragma unroll
+        for (int m = 0; m < M; m++) AD[M * J.out_offset + m * npix + pix] = ad[m];
+    }
+}
+
+template <int V>
+static void launch_multi_v(const JobTable &t, int M, hipStream_t st, SpectrumDev sp, float step, int atten, int air,
+                           float *I, float *P, float *AD) {
+    const int64_t n = t.m_count[V];
+    if (n <= 0) return;
+    const dim3 g((unsigned)((n + 3) / 4)), b(256);
+    const int2 *items = t.d_mitems + t.m_first[V];
+    switch (M) {
+#define DRR_CASE(MM) \
+    case MM: k_batch_multi<V, MM><<<g, b, 0, st>>>(t.d_mjobs, items, n, sp, step, atten, air, I, P, AD); break;
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+#undef DRR_CASE
+    }
+}
+
+// Every job of a table: the persistent single-volume march, then the
+// multi-volume groups.
+static void launch_all(const JobTable &t, int M, bool atten, int tw, hipStream_t st, SpectrumDev sp, float step,
+                       int air, float *I, float *P, float *AD) {
+    if (t.n_items > 0) launch_march(t, M, atten, tw, st, sp, step, air, I, P, AD);
+    launch_multi_v<2>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
+    launch_multi_v<3>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
+    launch_multi_v<4>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
+}
+
 static void batch_free(drr_batch *B) {
     if (!B) return;
     if (B->exec_march) (void)hipGraphExecDestroy(B->exec_march);
@@ -237,7 +349,7 @@ static hipError_t capture(drr_batch *B, SpectrumDev sp, float step, int air, flo
     hipError_t e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     k_jobs_reset<<<1, 64, 0, B->cap>>>(B->t.d_counters);
-    launch_march(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P, nullptr);
+    launch_all(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P, nullptr);
     e = hipStreamEndCapture(B->cap, &B->graph_march);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&B->exec_march, B->graph_march, nullptr, nullptr, 0);
@@ -353,8 +465,8 @@ extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);
-    launch_march(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
-                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
+    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
+               ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
     hipError_t e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;
@@ -414,10 +526,15 @@ extern "C" int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *v
     unsigned char *base = nullptr;
     int rc = scratch_address(ctx->device, &base);
     if (rc != DRR_OK) return rc;
-    // layout: counters | segments | volumes | jobs | tiles, each 256-B aligned
+    // layout: counters | segments | volumes | jobs | multi-volume jobs | tiles
+    // (single-volume, then multi-volume), each 256-B aligned; a job over
+    // several volumes takes room in both job tables
+    int n_multi = 0;
+    for (int j = 0; j < n_jobs; j++) n_multi += jobs[j].view.n_volumes > 1 ? 1 : 0;
     const size_t off_seg = 256, off_vols = 512;
     const size_t off_jobs = align256(off_vols + sizeof(VolDev) * (size_t)n_vols);
-    const size_t off_items = align256(off_jobs + sizeof(JobDev) * (size_t)n_jobs);
+    const size_t off_mjobs = align256(off_jobs + sizeof(JobDev) * (size_t)n_jobs);
+    const size_t off_items = align256(off_mjobs + sizeof(JobMultiDev) * (size_t)n_multi);
     if (off_items > DRR_NOALLOC_SCRATCH_BYTES)
         return drr_set_error(DRR_ERR_INVALID, "drr_project_jobs_noalloc: the job table exceeds the " +
                                                   std::to_string(DRR_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
@@ -425,13 +542,15 @@ extern "C" int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *v
     JobTable t;
     JobHost h;
     const drr_batch_tuning_t tune{8, 8, 4, 8};
-    rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, max_items, "drr_project_jobs_noalloc", t, h);
+    rc = jobs_prepare(ctx, vols, n_vols, jobs, n_jobs, tune, max_items, "drr_project_jobs_noalloc", true, t, h);
     if (rc != DRR_OK) return rc;
     t.d_counters = (int *)base;
     t.d_seg = (int *)(base + off_seg);
     t.d_vols = (VolDev *)(base + off_vols);
     t.d_jobs = (JobDev *)(base + off_jobs);
+    t.d_mjobs = (JobMultiDev *)(base + off_mjobs);
     t.d_items = (int2 *)(base + off_items);
+    t.d_mitems = t.d_items + h.items.size();
     hipStream_t st = (hipStream_t)stream;
     // The host tables must outlive the copies that read them: every return
     // after the first copy is queued synchronizes the stream first.
@@ -447,13 +566,18 @@ extern "C" int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *v
     NCHK(hipMemcpyAsync(t.d_seg, h.seg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
     NCHK(hipMemcpyAsync(t.d_vols, h.vols.data(), sizeof(VolDev) * n_vols, hipMemcpyHostToDevice, st));
     NCHK(hipMemcpyAsync(t.d_jobs, h.jobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, st));
-    NCHK(hipMemcpyAsync(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice, st));
+    if (!h.items.empty())
+        NCHK(hipMemcpyAsync(t.d_items, h.items.data(), sizeof(int2) * h.items.size(), hipMemcpyHostToDevice, st));
+    if (!h.mjobs.empty()) {
+        NCHK(hipMemcpyAsync(t.d_mjobs, h.mjobs.data(), sizeof(JobMultiDev) * h.mjobs.size(), hipMemcpyHostToDevice, st));
+        NCHK(hipMemcpyAsync(t.d_mitems, h.mitems.data(), sizeof(int2) * h.mitems.size(), hipMemcpyHostToDevice, st));
+    }
 #undef NCHK
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);  // the previous call's counters are left in scratch
-    launch_march(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
-                 ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
+    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
+               ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
     e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;

@@ -2,7 +2,8 @@
 // batched forward (batch.hip) and the batched adjoint (adjoint.hip).
 //
 // A job is one view of one volume; its detector is cut into tiles of 64
-// pixels (one wavefront).  The tile list is grouped by volume and job and
+// pixels (one wavefront).  (Views over several volumes are jobs of their own
+// table, marched by k_batch_multi in batch.hip.)  The tile list is grouped by volume and job and
 // split into 8 contiguous segments, one per XCD.  A persistent wave takes
 // tiles from its XCD's segment (atomic counter) and steals from the other
 // segments once it is empty, so each XCD's L2 serves a coherent range of
@@ -23,21 +24,36 @@ struct JobDev {
     int64_t out_offset;
 };
 
+// A job over 2..4 volumes: the literal multi-volume march (priorities, the
+// shared label cache) of the whole view, on 8x8-pixel wave tiles.
+struct JobMultiDev {
+    ViewDev vd;
+    int tiles_x;
+    int pad;
+    int64_t out_offset;
+};
+
 struct JobTable {
     int n_jobs = 0, n_vols = 0;
-    JobDev *d_jobs = nullptr;
+    JobDev *d_jobs = nullptr;   // single-volume jobs (entries of multi-volume jobs unused)
     VolDev *d_vols = nullptr;
-    int2 *d_items = nullptr;
+    int2 *d_items = nullptr;    // single-volume tiles (job, tile)
     int *d_seg = nullptr;       // 9 segment starts
     int *d_counters = nullptr;  // 8 queue counters
     int64_t n_items = 0, max_npix = 0;
     int grid = 0, block = 0;
     std::vector<ImgSeg> imgs;   // host copy: offset / size of each job's image
+    // multi-volume jobs: their own table, tiles grouped by volume count V
+    JobMultiDev *d_mjobs = nullptr;
+    int2 *d_mitems = nullptr;   // (index into d_mjobs, tile)
+    int n_mjobs = 0;
+    int64_t m_first[DRR_MAX_VOLUMES + 1] = {}, m_count[DRR_MAX_VOLUMES + 1] = {};
 };
 
 // Build (and upload) the job table; host-side validation included.
-// drr_jobs_build accepts the wave tiles of the batched march (8x8, 16x4);
-// drr_jobs_build_tiles any tile (the adjoint's 16x16 workgroup tiles).
+// drr_jobs_build accepts the wave tiles of the batched march (8x8, 16x4) and
+// jobs over several volumes; drr_jobs_build_tiles any tile (the adjoint's
+// 16x16 workgroup tiles) and single-volume jobs only.
 int drr_jobs_build(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,
                    int n_jobs, const drr_batch_tuning_t &tune, JobTable &t);
 int drr_jobs_build_tiles(drr_ctx *ctx, const drr_volume *const *vols, int n_vols, const drr_job_t *jobs,

@@ -445,8 +445,9 @@ class Projector:
         """All views in one persistent launch (hipGraph), device neglog, one D2H copy.
 
         Returns what ``project`` returns for the same views ([H, W] for one view,
-        [N, H, W] for equal sizes, else a list).  Each view renders one volume
-        (``volume_indices``; required when the projector holds several).
+        [N, H, W] for equal sizes, else a list).  Each view renders all the
+        projector's volumes, as ``project`` does, or, with ``volume_indices``,
+        the one volume given per view.
         """
         from .batch import plan_for_projector
 

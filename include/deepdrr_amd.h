@@ -190,11 +190,15 @@ int drr_mesh_rays(drr_ctx *ctx, const drr_mesh *mesh, const drr_view_t *view, co
 
 /* ---- batched forward (one persistent launch, hipGraph) ------------------- */
 
-/* One job of a batch: a view of a single volume written at `out_offset`
- * floats into the batch outputs. */
+/* One job of a batch: a view written at `out_offset` floats into the batch
+ * outputs.  A view of n_volumes volumes pairs view.vol[v] with the batch's
+ * volume volume_index + v (the reference renders every view over all its
+ * volumes, projector.py:679-685); views over several volumes run the
+ * multi-volume march (priorities, shared label cache) with the same results
+ * as drr_project.  The adjoint entry points take single-volume jobs only. */
 typedef struct {
-    drr_view_t view;           /* n_volumes must be 1 */
-    int32_t volume_index;      /* index into the batch's volume list */
+    drr_view_t view;           /* n_volumes in [1, 4] */
+    int32_t volume_index;      /* first of the view's volumes in the batch's volume list */
     int64_t out_offset;        /* element offset of this view's [H][W] image */
 } drr_job_t;
 

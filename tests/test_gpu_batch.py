@@ -197,3 +197,96 @@ def test_projector_batch_device_noise(gpu):
     inner = (slice(None), slice(2, -2), slice(2, -2))
     rel = abs(raw[inner].mean() - clean_raw[inner].mean()) / clean_raw[inner].mean()
     assert rel < 0.01, rel
+
+
+def test_multi_volume_jobs(gpu):
+    """Views over 2 and 3 volumes (with priorities and air outside) mixed with
+    single-volume views in one batch: each image bit-identical to drr_project
+    of the same view, under both tile shapes; drr_project_jobs' area densities
+    too."""
+    import torch
+
+    from deepdrr_amd._abi_types import Job
+
+    ctx, handles, arrays, _, _, sc, E, pdf, mu = _setup(n_views=3)
+    vols = [phantoms.small_ct(24, 0), phantoms.small_ct(16, 1), phantoms.small_ct(20, 2)]
+    vols[1].place_center([5.0, 3.0, -4.0])
+    try:
+        views, vidx = [], []
+        for i, (p, d, f) in enumerate(phantoms.batch_poses(9, seed=4, radius=6.0)):
+            w, h = [(12, 10), (24, 20), (17, 9)][i % 3]
+            proj = phantoms.pose_projection(w, h, 4.0 * 12 / w, p, d, f)
+            first, n = [(0, 2), (1, 2), (0, 3), (2, 1), (0, 1)][i % 5]
+            prios = [[1, 0], [0, 0], [2, 0, 1], [0], [0]][i % 5]
+            views.append(P.build_view(vols[first:first + n], prios, proj, 4080.0))
+            vidx.append(first)
+        print(f"multi-volume batch: {[v.n_volumes for v in views]} volumes per view", flush=True)
+        for tuning in ((8, 8, 4, 8), (16, 4, 2, 4)):
+            plan = BatchPlan(ctx, handles, views, vidx, neglog=False, tuning=tuning)
+            plan.launch()
+            torch.cuda.synchronize()
+            I_b = plan.fetch("intensity")
+            plan.close()
+            for i, (v, vi) in enumerate(zip(views, vidx)):
+                I = torch.empty((v.height, v.width), device=gpu)
+                Pp = torch.empty_like(I)
+                varr = (C.c_void_p * v.n_volumes)(*[handles[vi + k].value for k in range(v.n_volumes)])
+                _abi.call("drr_project", ctx, varr, C.byref(v), _abi.ptr(I), _abi.ptr(Pp), None, None,
+                          _abi.stream_ptr())
+                torch.cuda.synchronize()
+                assert np.array_equal(I.cpu().numpy().view(np.uint32), I_b[i].view(np.uint32)), (tuning, i)
+        # drr_project_jobs with area densities
+        M = sc.n_materials
+        jobs = (Job * len(views))()
+        off = 0
+        offs = []
+        for i, (v, vi) in enumerate(zip(views, vidx)):
+            jobs[i].view = v
+            jobs[i].volume_index = vi
+            jobs[i].out_offset = off
+            offs.append(off)
+            off += v.width * v.height
+        I_j = torch.empty(off, device=gpu)
+        AD_j = torch.empty(M * off, device=gpu)
+        _abi.call("drr_project_jobs", ctx, (C.c_void_p * 3)(*[h.value for h in handles]), 3, jobs, len(views),
+                  _abi.ptr(I_j), None, _abi.ptr(AD_j), _abi.stream_ptr())
+        torch.cuda.synchronize()
+        AD_all = AD_j.cpu().numpy()
+        for i, (v, vi) in enumerate(zip(views, vidx)):
+            I = torch.empty((v.height, v.width), device=gpu)
+            Pp = torch.empty_like(I)
+            AD = torch.empty((M, v.height, v.width), device=gpu)
+            varr = (C.c_void_p * v.n_volumes)(*[handles[vi + k].value for k in range(v.n_volumes)])
+            _abi.call("drr_project", ctx, varr, C.byref(v), _abi.ptr(I), _abi.ptr(Pp), _abi.ptr(AD), None,
+                      _abi.stream_ptr())
+            torch.cuda.synchronize()
+            npx = v.width * v.height
+            got = AD_all[M * offs[i]: M * offs[i] + M * npx].reshape(M, v.height, v.width)
+            assert np.array_equal(AD.cpu().numpy().view(np.uint32), got.view(np.uint32)), i
+        # a view whose volumes run past the batch's list is refused
+        jobs[0].volume_index = 2
+        rc = _abi.lib().drr_project_jobs(ctx, (C.c_void_p * 3)(*[h.value for h in handles]), 3, jobs, 1,
+                                         _abi.ptr(I_j), None, None, _abi.stream_ptr())
+        assert rc != 0 and "volume_index" in _abi.last_error()
+    finally:
+        _teardown(ctx, handles)
+
+
+def test_projector_batch_two_volumes(gpu):
+    """Projector.project_batch over a two-volume projector renders every view
+    over both volumes, as project() does (the reference's multi-volume scene,
+    tests/test_multivolume.py:19-32)."""
+    from deepdrr_amd import Projector
+
+    a = phantoms.small_ct(24, 0)
+    b = phantoms.small_ct(16, 1)
+    b.place_center([4.0, 2.0, -3.0])
+    dev = phantoms.ap_device(20, 16, 2.0)
+    projs = [phantoms.pose_projection(20, 16, 2.0, p, d, f)
+             for p, d, f in phantoms.batch_poses(4, seed=9, radius=4.0)]
+    with Projector([a, b], device=dev, spectrum=synthetic_120kv(), priorities=[1, 0]) as proj:
+        ref = proj(*projs)
+        got = proj.project_batch(*projs)
+        single = proj.project_batch(*projs, volume_indices=[0, 1, 0, 1])
+    np.testing.assert_allclose(got, ref, atol=1e-5)
+    assert not np.allclose(single, ref, atol=1e-3)

@@ -147,8 +147,11 @@ def test_job_view_and_plan_errors(gpu):
         J[0].out_offset = -1
         assert project_jobs(J, 1) != 0 and "offset" in _abi.last_error()
         J[0].out_offset = 0
-        J[0].view.n_volumes = 2
-        assert project_jobs(J, 1) != 0 and "exactly one volume" in _abi.last_error()
+        J[0].view.n_volumes = 2  # two volumes from a list of one
+        assert project_jobs(J, 1) != 0 and "volume_index" in _abi.last_error()
+        J[0].view.n_volumes = 5
+        assert project_jobs(J, 1) != 0 and "n_volumes" in _abi.last_error()
+        J[0].view.n_volumes = 1
         for field, val in (("n_volumes", 0), ("n_volumes", 5), ("width", 0), ("height", 65537)):
             v = type(case.view).from_buffer_copy(case.view)
             setattr(v, field, val)

@@ -40,11 +40,16 @@ def _scene():
     return ctx, handles, vols, len(mats)
 
 
-def _jobs(vols, n, seed, sizes):
+def _jobs(vols, n, seed, sizes, multi=False):
     views, vidx = [], []
     for i, (p, d, f) in enumerate(phantoms.batch_poses(n, seed=seed, radius=6.0)):
         w, h = sizes[i % len(sizes)]
         proj = phantoms.pose_projection(w, h, 48.0 / w, p, d, f, center=np.array(vols[i % 3].center_in_world))
+        if multi and i % 2:
+            # a view over volumes 1 and 2 (the multi-volume job table)
+            views.append(P.build_view(vols[1:3], [1, 0], proj, 4080.0))
+            vidx.append(1)
+            continue
         views.append(P.build_view([vols[i % 3]], [0], proj, 4080.0))
         vidx.append(i % 3)
     jobs = (Job * n)()
@@ -74,8 +79,8 @@ def test_noalloc_matches_allocating_entry_point(gpu):
     varr = (C.c_void_p * 3)(*[h.value for h in handles])
     try:
         # a larger batch, then a smaller one over the scratch the first left behind
-        for n, seed, sizes in [(12, 3, [(40, 33), (24, 20), (17, 9)]), (4, 8, [(9, 7), (16, 13)])]:
-            jobs, total = _jobs(vols, n, seed, sizes)
+        for n, seed, sizes, multi in [(12, 3, [(40, 33), (24, 20), (17, 9)], True), (4, 8, [(9, 7), (16, 13)], False)]:
+            jobs, total = _jobs(vols, n, seed, sizes, multi)
             print(f"noalloc: {n} jobs, {total} pixels", flush=True)
             ref = _run("drr_project_jobs", ctx, varr, 3, jobs, n, total, M)
             got = _run("drr_project_jobs_noalloc", ctx, varr, 3, jobs, n, total, M)
