@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 DRR_MAX_VOLUMES = 4
-DRR_MAX_MATERIALS = 8
+DRR_MAX_MATERIALS = 16
 DRR_MAX_BINS = 256
 DRR_MESH_MAX_LAYERS = 4
 DRR_MESH_MAX_HITS = 128

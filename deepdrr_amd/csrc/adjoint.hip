@@ -813,7 +813,8 @@ extern "C" int drr_adjoint(drr_ctx *ctx, const drr_volume *vol, const drr_view_t
         else                                                                                      \
             launch_adj<MM, false>(pose, grid, block, st, g, v, sp, sc.step, sc.air_index, a, sw, tx); \
         break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
     hipError_t e = hipGetLastError();
@@ -882,7 +883,8 @@ extern "C" int drr_adjoint_jobs(drr_ctx *ctx, const drr_volume *vol, const drr_j
         else                                                                                          \
             launch_adj_jobs<MM, false>(pose, t, st, d_sw, v, sp, sc.step, sc.air_index, a, d_pose64); \
         break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
     if (pose) k_pose_finish<<<1, 64, 0, st>>>(d_pose64, grad_pose);

@@ -265,7 +265,8 @@ static void launch_march(const JobTable &t, int M, bool atten, int tw, hipStream
         else                                                                     \
             launch_march_tw<MM, false>(tw, t, st, sp, step, air, I, P, AD);      \
         break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
 }
@@ -311,7 +312,8 @@ static void launch_multi_v(const JobTable &t, int M, hipStream_t st, SpectrumDev
     switch (M) {
 #define DRR_CASE(MM) \
     case MM: k_batch_multi<V, MM><<<g, b, 0, st>>>(t.d_mjobs, items, n, sp, step, atten, air, I, P, AD); break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
 }

@@ -6,7 +6,7 @@
 //   words 0-7   : 8 float32 corner densities, corner d = dx + 2 dy + 4 dz
 //                 holding D[clamp(i+dx)][clamp(j+dy)][clamp(k+dz)]   (32 B)
 //   word 8      : the 8 corner labels as 4-bit fields, corner d at bits
-//                 4d..4d+3 (labels are < DRR_MAX_MATERIALS = 8)
+//                 4d..4d+3 (labels are < DRR_MAX_MATERIALS = 16)
 //   word 9      : their common label if all 8 agree, else ~0          (8 B)
 // A ray sample with base = floor(p) reads cell clamp(base, -1, n-1): three
 // loads from one address (16 + 16 + 8 bytes) replace the 16 scattered texel

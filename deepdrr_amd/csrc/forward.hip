@@ -13,7 +13,7 @@ using namespace drr;
 
 // One thread per cell (i, j, k) in [-1, n-1]^3, i fastest: writes its 40-byte
 // record (drr_internal.h): the 8 clamped corner densities, the corner labels
-// as nibbles and their common label or ~0 (labels are < 8), as five 8-byte
+// as nibbles and their common label or ~0 (labels are < 16), as five 8-byte
 // stores.  Consecutive threads write consecutive records, so the stores are
 // fully coalesced.
 __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ density,
@@ -142,7 +142,8 @@ static void launch_multi(int M, dim3 g, dim3 b, hipStream_t st, const ViewDev &v
     switch (M) {
 #define DRR_CASE(MM) \
     case MM: k_project_multi<V, MM><<<g, b, 0, st>>>(vd, sp, step, atten, air, o, tx); break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
 }
@@ -172,7 +173,8 @@ extern "C" int drr_project(drr_ctx *ctx, const drr_volume *const *vols, const dr
         switch (sc.n_materials) {
 #define DRR_CASE(MM) \
     case MM: launch_v1<MM>(sc.attenuate_outside != 0, grid, block, st, geo, vol0, sp, sc.step, sc.air_index, o, tx); break;
-            DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+            DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
         }
     } else if (view->n_volumes == 2) {

@@ -395,7 +395,7 @@ extern "C" int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_
     if (vol->ctx != ctx) return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: volume of another context");
     const drr_mc_params_t &p = *params;
     if (p.n_materials < 1 || p.n_materials > DRR_MAX_MATERIALS || p.n_bins < 1 || p.n_bins > DRR_MAX_BINS)
-        return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: n_materials in [1,8] and n_bins in [1,256]");
+        return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: n_materials in [1,16] and n_bins in [1,256]");
     if (p.det_width < 1 || p.det_height < 1 || first_history < 0 || n_histories < 0)
         return drr_set_error(DRR_ERR_INVALID, "drr_mc_scatter: bad detector size or history range");
     if (!(p.cone_cos > -1.0f && p.cone_cos < 1.0f))

@@ -644,7 +644,8 @@ void launch_mesh_march(int M, dim3 g, dim3 b, hipStream_t st, const ViewDev &vd,
     switch (M) {
 #define DRR_CASE(MM) \
     case MM: k_project_mesh<V, MM><<<g, b, 0, st>>>(vd, sp, step, atten, air, mt, o, tx); break;
-        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8)
+        DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
+        DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE
     }
 }

@@ -110,7 +110,7 @@ int drr_ctx_set_scene(drr_ctx *ctx, const drr_scene_t *scene) {
     if (!(scene->step > 0.0f) || !std::isfinite(scene->step))
         return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: step must be a positive finite number");
     if (scene->n_materials < 1 || scene->n_materials > DRR_MAX_MATERIALS)
-        return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: n_materials must be in [1, 8]");
+        return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: n_materials must be in [1, 16]");
     if (scene->air_index < 0 || scene->air_index >= scene->n_materials)
         return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: air_index out of range");
     if (ctx->have_scene && ctx->n_bins > 0 && scene->n_materials != ctx->scene.n_materials) {

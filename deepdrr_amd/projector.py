@@ -200,7 +200,7 @@ class Projector:
 
         all_mats = []
         for _vol in self.volumes:
-            all_mats.extend(list(_vol.materials[0].keys()))
+            all_mats.extend(list(_vol.material_ids.keys()))
         for prim in self.primitives:
             all_mats.append(prim.material.drrMatName)
         if attenuate_outside_volume:
@@ -212,8 +212,8 @@ class Projector:
         else:
             self.air_index = 0
         self.attenuate_outside_volume = attenuate_outside_volume
-        if len(self.all_materials) > 8:
-            raise ValueError("at most 8 distinct materials are supported")
+        if len(self.all_materials) > 16:
+            raise ValueError("at most 16 distinct materials are supported")
         for mat in self.all_materials:
             try:
                 Material.from_string(mat)
@@ -282,13 +282,25 @@ class Projector:
                       self.energies.ctypes.data, self.pdf.ctypes.data,
                       self.absorption_coef_table.ctypes.data)
             for _vol in self.volumes:
-                dens, seg = volume_arrays(_vol, self.all_materials)
-                if not np.all(np.isfinite(dens)):
-                    raise ValueError("volume density must be finite")
                 h = C.c_void_p()
-                nz, ny, nx = dens.shape
-                _abi.call("drr_volume_create", self._ctx, dens.ctypes.data, seg.ctypes.data,
-                          nx, ny, nz, 0, C.byref(h))
+                src = _vol.hu_source()
+                if src is not None:
+                    # int16 HU: density and labels derived on the GPU
+                    # (volume.py:338-351, load_dicom.py:132-143)
+                    hu, classes = src
+                    hu_kji = np.ascontiguousarray(np.moveaxis(hu, [0, 1, 2], [2, 1, 0]))
+                    upper = np.array([u for _, u in classes], np.float32)
+                    lab = np.array([self.all_materials.index(n) for n, _ in classes], np.uint8)
+                    nz, ny, nx = hu_kji.shape
+                    _abi.call("drr_volume_create_from_hu", self._ctx, hu_kji.ctypes.data, nx, ny, nz, 0,
+                              len(upper), upper.ctypes.data, lab.ctypes.data, C.byref(h))
+                else:
+                    dens, seg = volume_arrays(_vol, self.all_materials)
+                    if not np.all(np.isfinite(dens)):
+                        raise ValueError("volume density must be finite")
+                    nz, ny, nx = dens.shape
+                    _abi.call("drr_volume_create", self._ctx, dens.ctypes.data, seg.ctypes.data,
+                              nx, ny, nz, 0, C.byref(h))
                 self._vols.append(h)
         except Exception:
             self._release()

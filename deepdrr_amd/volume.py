@@ -43,6 +43,13 @@ def segment_by_thresholds(hu_values: np.ndarray,
     return out
 
 
+# conv_hu_to_materials_thresholding as classes for segment_by_thresholds
+THRESHOLD_CLASSES = (
+    ("air", -800.0),
+    ("soft tissue", 350.0),
+    ("bone", np.inf),
+)
+
 FIVE_MATERIAL_CLASSES = (
     ("air", -900.0),
     ("lung", -200.0),
@@ -98,16 +105,72 @@ class Volume:
         self.anatomical_from_IJK = geo.frame_transform(anatomical_from_IJK)
         self.world_from_anatomical = (geo.FrameTransform.identity(3) if world_from_anatomical is None
                                       else geo.frame_transform(world_from_anatomical))
-        data = np.asarray(data)
-        assert data.ndim == 3, "Volume data must be 3D."
-        self.data = np.array(data).astype(np.float32)
-        if isinstance(materials, tuple):
-            self.materials = materials[0], np.asarray(materials[1]).astype(np.uint16)
+        hu_source = kwargs.pop("_hu_source", None)
+        self._hu = None
+        self._hu_classes = None
+        if hu_source is not None:
+            # HU held as int16: density and labels are derived when first read
+            # on the host, or on the GPU by the projector (drr_volume_create_from_hu)
+            self._hu, self._hu_classes = hu_source
+            self._data = None
+            self._materials = None
+            self._material_ids = {name: i for i, (name, _) in enumerate(self._hu_classes)}
         else:
-            self.materials = self._format_materials(materials)
+            data = np.asarray(data)
+            assert data.ndim == 3, "Volume data must be 3D."
+            self._data = np.array(data).astype(np.float32)
+            if isinstance(materials, tuple):
+                self._materials = materials[0], np.asarray(materials[1]).astype(np.uint16)
+            else:
+                self._materials = self._format_materials(materials)
+            self._material_ids = self._materials[0]
         self.anatomical_coordinate_system = anatomical_coordinate_system
         assert self.anatomical_coordinate_system in ["LPS", "RAS", None]
         self.enabled = enabled
+
+    # -- voxel data (derived from HU on first host access) ---------------------
+    def _materialize(self) -> None:
+        hu = self._hu
+        self._data = self._convert_hounsfield_to_density(hu).astype(np.float32)
+        self._materials = self._format_materials(segment_by_thresholds(hu, self._hu_classes))
+
+    @property
+    def data(self) -> np.ndarray:
+        if self._data is None:
+            self._materialize()
+        return self._data
+
+    @data.setter
+    def data(self, value) -> None:
+        if self._data is None and self._hu is not None:
+            self._materialize()
+        self._data = np.asarray(value, dtype=np.float32)
+
+    @property
+    def materials(self):
+        if self._materials is None:
+            self._materialize()
+        return self._materials
+
+    @materials.setter
+    def materials(self, value) -> None:
+        if self._data is None and self._hu is not None:
+            self._materialize()
+        self._materials = value
+        self._material_ids = value[0]
+
+    @property
+    def material_ids(self) -> Dict[str, int]:
+        """Material name -> id of ``materials`` (without deriving the labels)."""
+        return self._material_ids
+
+    def hu_source(self):
+        """(int16 HU [i, j, k], ((name, upper HU), ...)) while the density and
+        labels have not been derived on the host (so nothing can have changed
+        them), else None."""
+        if self._hu is None or self._data is not None:
+            return None
+        return self._hu, self._hu_classes
 
     # -- construction -------------------------------------------------------
     @classmethod
@@ -133,13 +196,25 @@ class Volume:
     def from_hu(cls, hu_values, origin, use_thresholding: bool = True, spacing=(1, 1, 1),
                 anatomical_coordinate_system: Optional[str] = None,
                 world_from_anatomical=None, material_classes=None, **kwargs) -> "Volume":
-        data = cls._convert_hounsfield_to_density(hu_values)
+        """vol/volume.py:280-302.  An int16 HU array (the on-disk CT format) is
+        kept as is: density and labels are derived on first host access, or
+        on the GPU when a projector uploads the volume."""
         if material_classes is None:
             if not use_thresholding:
                 raise NotImplementedError("network segmentation is not part of this engine")
-            materials = conv_hu_to_materials_thresholding(hu_values)
+            classes = THRESHOLD_CLASSES
         else:
-            materials = segment_by_thresholds(hu_values, material_classes)
+            classes = tuple((str(n), float(u)) for n, u in material_classes)
+        hu = np.asarray(hu_values)
+        uppers = [u for _, u in classes]
+        partition = all(a < b for a, b in zip(uppers, uppers[1:])) and uppers[-1] == np.inf
+        if hu.ndim == 3 and hu.dtype == np.int16 and partition:
+            return cls.from_parameters(None, None, origin=origin, spacing=spacing,
+                                       anatomical_coordinate_system=anatomical_coordinate_system,
+                                       world_from_anatomical=world_from_anatomical,
+                                       _hu_source=(np.ascontiguousarray(hu), classes), **kwargs)
+        data = cls._convert_hounsfield_to_density(hu)
+        materials = segment_by_thresholds(hu, classes)
         return cls.from_parameters(data, materials, origin=origin, spacing=spacing,
                                    anatomical_coordinate_system=anatomical_coordinate_system,
                                    world_from_anatomical=world_from_anatomical, **kwargs)
@@ -182,7 +257,7 @@ class Volume:
 
     @property
     def shape(self) -> Tuple[int, int, int]:
-        return self.data.shape
+        return tuple(self._hu.shape) if self._data is None else self._data.shape
 
     @property
     def spacing(self) -> geo.Vector3D:

@@ -33,7 +33,7 @@ extern "C" {
 #define DRR_ERR_STATE (-5)
 
 #define DRR_MAX_VOLUMES 4
-#define DRR_MAX_MATERIALS 8
+#define DRR_MAX_MATERIALS 16
 #define DRR_MAX_BINS 256
 
 typedef struct drr_ctx_s drr_ctx;
@@ -97,7 +97,8 @@ int drr_ctx_set_spectrum(drr_ctx *ctx, int n_bins, const float *energies_keV,
  * `on_device` = 1 if both pointers are HIP device pointers.  The volume is
  * stored cell-expanded in HBM: for every cell (i, j, k) in [-1, n-1]^3 one
  * 40-byte record of the 8 clamped corner densities (32 B), the 8 corner
- * labels as 4-bit fields and their common label (8 B).
+ * labels as 4-bit fields (so at most 16 materials) and their common label
+ * (8 B).
  * Replaces create_cuda_texture (projector.py:116-257, 1464-1539). */
 int drr_volume_create(drr_ctx *ctx, const float *density, const uint8_t *labels,
                       int nx, int ny, int nz, int on_device, drr_volume **out);

@@ -143,7 +143,7 @@ def mesh_rays(view: View, mesh, n_layers: int, max_hits: int, n_materials: int, 
     K = n_layers * max_hits // 2
     iv = np.zeros((H, W, K, 2), np.float32)
     cnt = np.zeros((H, W), np.int32)
-    add = np.zeros((n_layers * 8, H, W), np.float32)
+    add = np.zeros((n_layers * 16, H, W), np.float32)
     src = np.ascontiguousarray(source_world, np.float32)
     nu = lib().ora_mesh_rays(C.byref(view), verts.ctypes.data, tris.ctypes.data, tprim.ctypes.data, len(tris),
                              prims, n_prims, n_layers, max_hits, n_materials, src.ctypes.data, float(sdd),

@@ -214,7 +214,7 @@ class GpuRunner:
                 K = mc.n_layers * mc.max_hits // 2
                 iv = torch.full((H, W, K, 2), float("nan"), device=dev)
                 cnt = torch.full((H, W), -1, dtype=torch.int32, device=dev)
-                add = torch.full((mc.n_layers * 8, H, W), float("nan"), device=dev)
+                add = torch.full((mc.n_layers * 16, H, W), float("nan"), device=dev)
                 nu = C.c_int()
                 self.abi.call("drr_mesh_rays", self.ctx, h, C.byref(view), src.ctypes.data, C.c_float(mc.sdd),
                               self.abi.ptr(iv), self.abi.ptr(cnt), self.abi.ptr(add), C.byref(nu),

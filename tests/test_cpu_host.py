@@ -136,6 +136,21 @@ def test_projector_constructor_errors():
         Projector(vol, max_mesh_hits=6)
     with pytest.raises(KeyError):
         Projector(vol, spectrum="100KV")
+    # 16 materials are rendered (4-bit labels); a 17th is refused
+    from deepdrr_amd.volume import Volume
+
+    names = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon", "iron", "lead",
+             "copper", "polyethylene", "concrete", "Al", "Ca", "W", "Au"]
+    lab = np.arange(17 * 8).reshape(17, 8, 1) % 17
+    dens = np.ones(lab.shape, np.float32)
+    for k in (16, 17):
+        v = Volume.from_parameters(dens, {m: lab == i for i, m in enumerate(names[:k])}, origin=[0, 0, 0],
+                                   anatomical_coordinate_system="LPS")
+        if k == 16:
+            assert len(Projector(v).all_materials) == 16
+        else:
+            with pytest.raises(ValueError, match="16"):
+                Projector(v)
     p = Projector(vol)
     assert p.all_materials == ["air", "bone", "soft tissue"]
     with pytest.raises(RuntimeError):

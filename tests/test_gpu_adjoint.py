@@ -280,12 +280,13 @@ def test_adjoint_jobs_equals_sum_of_single_view_adjoints(gpu, n, det, n_views, s
         dp.close()
 
 
-ALL_MATS = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon"]
+ALL_MATS = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon", "iron", "lead",
+            "copper", "polyethylene", "concrete", "Al", "Ca", "W"]
 
 
-@pytest.mark.parametrize("k", [2, 4, 6, 8])
+@pytest.mark.parametrize("k", [2, 4, 6, 8, 16])
 def test_adjoint_material_counts(gpu, k):
-    """k materials (the adjoint kernels are instantiated for 1 to 8) labelled in
+    """k materials (the adjoint kernels are instantiated for 1 to 16) labelled in
     2x2x2 blocks, so that half the cells mix labels and the pose gradient's
     label term is exercised: density and pose gradients against dense FP64
     autograd of the same model within 1e-4 (relative L2)."""

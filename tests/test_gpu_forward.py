@@ -297,11 +297,12 @@ def test_clinical_size_band(gpu):
         r.close()
 
 
-MATERIAL_NAMES = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon"]
+MATERIAL_NAMES = ["air", "bone", "soft tissue", "lung", "titanium", "muscle", "blood", "teflon", "iron", "lead",
+                  "copper", "polyethylene", "concrete", "Al", "Ca", "W"]
 
 
 def _materials_scene(seed, k):
-    """k materials (1 to 8) over a small anisotropic volume: 3x3x3 blocks of one
+    """k materials (1 to 16) over a small anisotropic volume: 3x3x3 blocks of one
     label (uniform cells, material switches) with 10% of the voxels relabelled at
     random (mixed cells), two random poses, a random detector and step."""
     from deepdrr_amd import phantoms
@@ -328,10 +329,10 @@ def _materials_scene(seed, k):
     return case, vol, projs
 
 
-@pytest.mark.parametrize("k", range(1, 9))
+@pytest.mark.parametrize("k", list(range(1, 9)) + [11, 16])
 def test_material_counts(gpu, k):
-    """Every material count the marches are instantiated for (1 to 8; from 6 on
-    the marches run at a lower occupancy): bit-exact area densities against the
+    """Material counts the marches are instantiated for (1 to 16, as 4-bit
+    labels; from 6 on the marches run at a lower occupancy): bit-exact area densities against the
     transcription, and batched images bit-identical to one view per launch with
     both tile shapes."""
     import torch

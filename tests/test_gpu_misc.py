@@ -169,3 +169,38 @@ def test_job_view_and_plan_errors(gpu):
         assert np.all(np.isfinite(g[0]))
     finally:
         r.close()
+
+
+def test_projector_ingests_int16_hu_on_gpu(gpu):
+    """A Volume made from int16 HU keeps the HU; Projector.initialize derives
+    density and labels on the GPU (drr_volume_create_from_hu) without ever
+    computing them on the host, and renders bit-identically to the same volume
+    whose density and labels were derived on the host."""
+    from deepdrr_amd import Projector, phantoms
+    from deepdrr_amd.spectral_data import synthetic_120kv
+    from deepdrr_amd.volume import Volume
+
+    hu = phantoms.clinical_ct_hu((48, 44, 40), seed=3)
+    kw = dict(origin=phantoms._centered_origin(hu.shape, (1.0, 1.0, 1.2)), spacing=(1.0, 1.0, 1.2),
+              anatomical_coordinate_system="LPS", material_classes=phantoms.FIVE_MATERIAL_CLASSES)
+    lazy = Volume.from_hu(hu, **kw)
+    host = Volume.from_hu(hu, **kw)
+    _ = host.data  # derived on the host
+    assert lazy.hu_source() is not None and host.hu_source() is None
+    dev = phantoms.ap_device(40, 36, 1.5)
+    outs = []
+    for vol in (lazy, host):
+        with Projector(vol, device=dev, spectrum=synthetic_120kv(), neglog=False) as p:
+            outs.append(p())
+    assert lazy._data is None, "the lazy volume was derived on the host"
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    # default thresholds (air/soft tissue/bone) as well
+    lazy3 = Volume.from_hu(hu, origin=kw["origin"], spacing=kw["spacing"], anatomical_coordinate_system="LPS")
+    host3 = Volume.from_hu(hu, origin=kw["origin"], spacing=kw["spacing"], anatomical_coordinate_system="LPS")
+    _ = host3.materials  # derived on the host
+    assert lazy3.hu_source() is not None and host3.hu_source() is None
+    o3 = []
+    for vol in (lazy3, host3):
+        with Projector(vol, device=dev, spectrum=synthetic_120kv(), neglog=False) as p:
+            o3.append(p())
+    assert np.array_equal(o3[0].view(np.uint32), o3[1].view(np.uint32))
