@@ -319,7 +319,18 @@ struct AdjRay {
         // one branch-free continuation test, evaluated at the bottom of the loop
         bool cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
         while (cont) {
-            if (!same) {
+            // A lane whose next sample lies in another cell waits until at
+            // least half of the lanes still marching wait too (or all of them
+            // do), and then they change cells together.  The hand-over block
+            // costs the whole wave its full length whenever any lane runs it,
+            // and some lane leaves its cell on most samples, so running it for
+            // many lanes at once saves most of its executions.  Waiting only
+            // delays a lane's next sample; every lane's sequence of samples,
+            // sums and hand-overs is unchanged.
+            const uint64_t marching = __ballot(1);
+            const uint64_t waiting = __ballot(!same);
+            const bool change_now = 2 * __popcll(waiting) >= __popcll(marching);
+            if (!same && change_now) {
                 const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
                 // one step to a face neighbour (in clamped cell coordinates): carry
                 // the shared face (density-only kernel); anything else: hand over
@@ -342,52 +353,55 @@ struct AdjRay {
                 fx = px - bx;
                 fy = py - by;
                 fz = pz - bz;
+                same = true;
             }
-            float w[8];
-            weights8(fx, fy, fz, w);
-            // t == 0 or t == N-1 (t < N), as one unsigned comparison
-            const float wt = ((uint32_t)(t - 1) >= (uint32_t)(N - 2)) ? 0.5f : 1.0f;
-            float sw;
-            if (c.uni) {
-                const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
-                sw = hu * ssum;
-            } else {
-                float h[8];
-                corner_h(h);
-                sw = 0.0f;
+            if (same) {
+                float w[8];
+                weights8(fx, fy, fz, w);
+                // t == 0 or t == N-1 (t < N), as one unsigned comparison
+                const float wt = ((uint32_t)(t - 1) >= (uint32_t)(N - 2)) ? 0.5f : 1.0f;
+                float sw;
+                if (c.uni) {
+                    const float ssum = ((((((w[0] + w[1]) + w[2]) + w[3]) + w[4]) + w[5]) + w[6]) + w[7];
+                    sw = hu * ssum;
+                } else {
+                    float h[8];
+                    corner_h(h);
+                    sw = 0.0f;
 #pragma unroll
-                for (int d = 0; d < 8; d++) sw = fmaf(w[d], h[d], sw);
-                if (POSE) {
-                    // label-gradient term of a cell whose labels differ (rare)
-                    const float rho = trilerp(c.d0, c.d1, w);
-                    float gh[3];
-                    grad_trilinear(h, fx, fy, fz, gh);
-                    const float wr = wt * rho;
+                    for (int d = 0; d < 8; d++) sw = fmaf(w[d], h[d], sw);
+                    if (POSE) {
+                        // label-gradient term of a cell whose labels differ (rare)
+                        const float rho = trilerp(c.d0, c.d1, w);
+                        float gh[3];
+                        grad_trilinear(h, fx, fy, fz, gh);
+                        const float wr = wt * rho;
 #pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        add_S(k, (double)(wr * gh[k]));
-                        add_S(k + 3, (double)(alpha * wr * gh[k]));
+                        for (int k = 0; k < 3; k++) {
+                            add_S(k, (double)(wr * gh[k]));
+                            add_S(k + 3, (double)(alpha * wr * gh[k]));
+                        }
                     }
                 }
-            }
-            const float G = wt * sw;
+                const float G = wt * sw;
 #pragma unroll
-            for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
-            if (POSE) {
-                const float aG = alpha * G;
+                for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
+                if (POSE) {
+                    const float aG = alpha * G;
 #pragma unroll
-                for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
+                    for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
+                }
+                alpha += step;
+                ++t;
+                px = sample_pos(sx, alpha, rix);
+                py = sample_pos(sy, alpha, riy);
+                pz = sample_pos(sz, alpha, riz);
+                fx = px - c.bx;
+                fy = py - c.by;
+                fz = pz - c.bz;
+                same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
+                cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
             }
-            alpha += step;
-            ++t;
-            px = sample_pos(sx, alpha, rix);
-            py = sample_pos(sy, alpha, riy);
-            pz = sample_pos(sz, alpha, riz);
-            fx = px - c.bx;
-            fy = py - c.by;
-            fz = pz - c.bz;
-            same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
-            cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
         }
         if (!(t < N && alpha <= r.mxv)) {
             flush(vol, dep);
