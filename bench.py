@@ -42,14 +42,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_ctx(_abi, materials, step):
-    from deepdrr_amd._abi_types import Scene
+def make_ctx(_abi, materials, step, march="fast"):
+    from deepdrr_amd._abi_types import MARCH_EXACT, MARCH_FAST, Scene
     from deepdrr_amd.material import attenuation_table
     from deepdrr_amd.spectral_data import spectrum_tables, synthetic_120kv
 
     ctx = C.c_void_p()
     _abi.call("drr_ctx_create", 0, C.byref(ctx))
-    sc = Scene(step=step, n_materials=len(materials), attenuate_outside=0, air_index=0)
+    sc = Scene(step=step, n_materials=len(materials), attenuate_outside=0, air_index=0,
+               march=MARCH_FAST if march == "fast" else MARCH_EXACT)
     _abi.call("drr_ctx_set_scene", ctx, C.byref(sc))
     E, pdf = spectrum_tables(synthetic_120kv())
     mu = attenuation_table(materials, E)
@@ -304,6 +305,9 @@ def main():
     ap.add_argument("--tuning", type=int, nargs=4, default=None,
                     help="tile_w tile_h waves_per_block blocks_per_cu (default: autotuned)")
     ap.add_argument("--step-mm", type=float, default=0.1)
+    ap.add_argument("--march", choices=("fast", "exact"), default="fast",
+                    help="single-volume march: fast (nested lerps, images within 1e-5 of the "
+                         "transcription) or exact (area densities bit-identical to it)")
     ap.add_argument("--plain-views", type=int, default=3)
     ap.add_argument("--mc-histories", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -358,7 +362,7 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.init()
     materials = sorted(workload.CLINICAL_CLASSES)
-    ctx, (E, pdf, mu, scene) = make_ctx(_abi, materials, args.step_mm)
+    ctx, (E, pdf, mu, scene) = make_ctx(_abi, materials, args.step_mm, args.march)
     log("generating the 512x512x400 phantom")
     t = time.perf_counter()
     vh, hu = workload.clinical_volume_device(ctx, materials)
@@ -411,6 +415,24 @@ def main():
     plan.close()
     value = B * args.steps / dt
     kernel_ms = float(np.mean(kms))
+    # the same batch marched by the other march (same schedule), for comparison
+    from deepdrr_amd._abi_types import MARCH_EXACT, MARCH_FAST
+
+    other = "exact" if args.march == "fast" else "fast"
+    scene.march = MARCH_EXACT if other == "exact" else MARCH_FAST
+    _abi.call("drr_ctx_set_scene", ctx, C.byref(scene))
+    plan_o = BatchPlan(ctx, [vh], views, [0] * B, neglog=True, tuning=tuple(plan.tuning))
+    kms_o = []
+    for i in range(3):
+        plan_o.launch()
+        torch.cuda.synchronize()
+        if i:
+            kms_o.append(plan_o.march_ms())
+    plan_o.close()
+    scene.march = MARCH_FAST if args.march == "fast" else MARCH_EXACT
+    _abi.call("drr_ctx_set_scene", ctx, C.byref(scene))
+    other_march_ms = float(np.mean(kms_o))
+    log(f"{other} march of the same batch: {other_march_ms / B:.2f} ms/view")
     gather_gbs = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
     traffic, traffic_tuning, valu, l2_hit = profiled_counters(B)
     # HBM bytes the launch moves (rocprof FETCH_SIZE + WRITE_SIZE of the same
@@ -429,8 +451,10 @@ def main():
                             "device neglog, one persistent launch per step replayed from a hipGraph",
                 "views_per_step": B, "tuning": list(plan.tuning),
                 "tuning_source": "given" if args.tuning else "autotuned"},
+        march=args.march,
         plain_views_per_s=plain_vps, plain_march_ms_per_view=plain_kernel_ms,
         batched_march_ms_per_view=kernel_ms / B, batched_over_plain=value / plain_vps,
+        **{f"batched_{other}_march_ms_per_view": other_march_ms / B},
         plain_samples_per_view=n_samples_ap,
         plain_march_samples_per_s=n_samples_ap / (plain_kernel_ms * 1e-3),
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

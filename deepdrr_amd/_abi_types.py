@@ -35,12 +35,17 @@ class View(C.Structure):
     ]
 
 
+MARCH_EXACT = 0  # DRR_MARCH_EXACT
+MARCH_FAST = 1  # DRR_MARCH_FAST
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("step", C.c_float),
         ("n_materials", C.c_int32),
         ("attenuate_outside", C.c_int32),
         ("air_index", C.c_int32),
+        ("march", C.c_int32),  # MARCH_EXACT or MARCH_FAST
     ]
 
 

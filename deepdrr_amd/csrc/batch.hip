@@ -207,6 +207,7 @@ struct drr_batch_s {
     drr_ctx *ctx = nullptr;
     int M = 0;
     bool atten = false;
+    bool fast = false;  // DRR_MARCH_FAST at creation
     drr_batch_tuning_t tune{};
     JobTable t;
     ImgSeg *d_imgs = nullptr;
@@ -218,7 +219,7 @@ struct drr_batch_s {
     bool launched = false;
 };
 
-template <int M, bool ATTEN, int TW>
+template <int M, bool ATTEN, int TW, bool FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M + (ATTEN ? 1 : 0)>::waves)))
 void k_batch_march(const JobDev *__restrict__ jobs, const VolDev *__restrict__ vols,
                    const int2 *__restrict__ items, const int *__restrict__ seg, int *counters,
@@ -227,7 +228,7 @@ void k_batch_march(const JobDev *__restrict__ jobs, const VolDev *__restrict__ v
     for_each_tile<TW>(jobs, items, seg, counters, [&](int, const JobDev &J, int udx, int vdx, bool valid) {
         if (!valid) return;
         float ad[M];
-        march_v1<M, ATTEN>(J.g, vols[J.vol], udx, vdx, step, air, ad);
+        march_v1<M, ATTEN, FAST>(J.g, vols[J.vol], udx, vdx, step, air, ad);
         float inten, pp;
         spectral<M>(sp, ad, inten, pp);
         const int64_t pix = (int64_t)vdx * J.g.width + udx;
@@ -243,27 +244,40 @@ void k_batch_march(const JobDev *__restrict__ jobs, const VolDev *__restrict__ v
     });
 }
 
-template <int M, bool ATTEN>
+template <int M, bool ATTEN, bool FAST>
 static void launch_march_tw(int tw, const JobTable &t, hipStream_t st, SpectrumDev sp, float step, int air,
                             float *I, float *P, float *AD) {
     dim3 g(t.grid), b(t.block);
     if (tw == 16)
-        k_batch_march<M, ATTEN, 16><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
-                                                      step, air, I, P, AD);
+        k_batch_march<M, ATTEN, 16, FAST><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters,
+                                                            sp, step, air, I, P, AD);
     else
-        k_batch_march<M, ATTEN, 8><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters, sp,
-                                                     step, air, I, P, AD);
+        k_batch_march<M, ATTEN, 8, FAST><<<g, b, 0, st>>>(t.d_jobs, t.d_vols, t.d_items, t.d_seg, t.d_counters,
+                                                           sp, step, air, I, P, AD);
 }
 
-static void launch_march(const JobTable &t, int M, bool atten, int tw, hipStream_t st, SpectrumDev sp,
-                         float step, int air, float *I, float *P, float *AD) {
+template <int M>
+static void launch_march_m(bool atten, bool fast, int tw, const JobTable &t, hipStream_t st, SpectrumDev sp,
+                           float step, int air, float *I, float *P, float *AD) {
+    if (fast) {
+        if (atten)
+            launch_march_tw<M, true, true>(tw, t, st, sp, step, air, I, P, AD);
+        else
+            launch_march_tw<M, false, true>(tw, t, st, sp, step, air, I, P, AD);
+    } else {
+        if (atten)
+            launch_march_tw<M, true, false>(tw, t, st, sp, step, air, I, P, AD);
+        else
+            launch_march_tw<M, false, false>(tw, t, st, sp, step, air, I, P, AD);
+    }
+}
+
+static void launch_march(const JobTable &t, int M, bool atten, bool fast, int tw, hipStream_t st,
+                         SpectrumDev sp, float step, int air, float *I, float *P, float *AD) {
     switch (M) {
 #define DRR_CASE(MM)                                                             \
     case MM:                                                                     \
-        if (atten)                                                               \
-            launch_march_tw<MM, true>(tw, t, st, sp, step, air, I, P, AD);       \
-        else                                                                     \
-            launch_march_tw<MM, false>(tw, t, st, sp, step, air, I, P, AD);      \
+        launch_march_m<MM>(atten, fast, tw, t, st, sp, step, air, I, P, AD);     \
         break;
         DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
         DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
@@ -320,9 +334,9 @@ static void launch_multi_v(const JobTable &t, int M, hipStream_t st, SpectrumDev
 
 // Every job of a table: the persistent single-volume march, then the
 // multi-volume groups.
-static void launch_all(const JobTable &t, int M, bool atten, int tw, hipStream_t st, SpectrumDev sp, float step,
-                       int air, float *I, float *P, float *AD) {
-    if (t.n_items > 0) launch_march(t, M, atten, tw, st, sp, step, air, I, P, AD);
+static void launch_all(const JobTable &t, int M, bool atten, bool fast, int tw, hipStream_t st, SpectrumDev sp,
+                       float step, int air, float *I, float *P, float *AD) {
+    if (t.n_items > 0) launch_march(t, M, atten, fast, tw, st, sp, step, air, I, P, AD);
     launch_multi_v<2>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
     launch_multi_v<3>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
     launch_multi_v<4>(t, M, st, sp, step, atten ? 1 : 0, air, I, P, AD);
@@ -351,7 +365,7 @@ static hipError_t capture(drr_batch *B, SpectrumDev sp, float step, int air, flo
     hipError_t e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     k_jobs_reset<<<1, 64, 0, B->cap>>>(B->t.d_counters);
-    launch_all(B->t, B->M, B->atten, B->tune.tile_w, B->cap, sp, step, air, I, P, nullptr);
+    launch_all(B->t, B->M, B->atten, B->fast, B->tune.tile_w, B->cap, sp, step, air, I, P, nullptr);
     e = hipStreamEndCapture(B->cap, &B->graph_march);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&B->exec_march, B->graph_march, nullptr, nullptr, 0);
@@ -393,6 +407,7 @@ extern "C" int drr_batch_create(drr_ctx *ctx, const drr_volume *const *vols, int
     B->ctx = ctx;
     B->M = ctx->scene.n_materials;
     B->atten = ctx->scene.attenuate_outside != 0;
+    B->fast = ctx->scene.march == DRR_MARCH_FAST;
     B->tune = tuning ? *tuning : drr_batch_tuning_t{8, 8, 4, 8};
     int rc = drr_jobs_build(ctx, vols, n_vols, jobs, n_jobs, B->tune, B->t);
     if (rc != DRR_OK) {
@@ -467,8 +482,8 @@ extern "C" int drr_project_jobs(drr_ctx *ctx, const drr_volume *const *vols, int
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);
-    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
-               ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
+    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, ctx->scene.march == DRR_MARCH_FAST,
+               tune.tile_w, st, sp, ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
     hipError_t e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;
@@ -578,8 +593,8 @@ extern "C" int drr_project_jobs_noalloc(drr_ctx *ctx, const drr_volume *const *v
     SpectrumDev sp{ctx->d_energies, ctx->d_pdf, ctx->d_mu, ctx->n_bins};
     (void)hipEventRecord(ctx->ev0, st);
     k_jobs_reset<<<1, 64, 0, st>>>(t.d_counters);  // the previous call's counters are left in scratch
-    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, tune.tile_w, st, sp,
-               ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
+    launch_all(t, ctx->scene.n_materials, ctx->scene.attenuate_outside != 0, ctx->scene.march == DRR_MARCH_FAST,
+               tune.tile_w, st, sp, ctx->scene.step, ctx->scene.air_index, intensity, photon_prob, area_density);
     e = hipGetLastError();
     (void)hipEventRecord(ctx->ev1, st);
     ctx->timed = true;

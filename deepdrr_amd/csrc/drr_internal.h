@@ -36,7 +36,7 @@ struct drr_volume_s {
 
 struct drr_ctx_s {
     int device = 0;
-    drr_scene_t scene{0.1f, 0, 0, 0};
+    drr_scene_t scene{0.1f, 0, 0, 0, DRR_MARCH_EXACT};
     bool have_scene = false;
     int n_bins = 0;
     float *d_energies = nullptr;  // [B]

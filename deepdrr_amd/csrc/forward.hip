@@ -77,7 +77,7 @@ struct OutDev {
     float *solid_angle;   // [H][W] or NULL
 };
 
-template <int M, bool ATTEN>
+template <int M, bool ATTEN, bool FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M>::waves))) void k_project_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step,
                                                     int air, OutDev out, int tiles16_x) {
     const int lb = blockIdx.x;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MarchOcc<M>
     lane_pixel(bx * 2 + (wave & 1), by * 2 + (wave >> 1), lane, udx, vdx);
     if (udx >= g.width || vdx >= g.height) return;
     float ad[M];
-    march_v1<M, ATTEN>(g, vol, udx, vdx, step, air, ad);
+    march_v1<M, ATTEN, FAST>(g, vol, udx, vdx, step, air, ad);
     float I, P;
     spectral<M>(sp, ad, I, P);
     const int64_t npix = (int64_t)g.width * g.height;
@@ -127,12 +127,19 @@ __global__ __launch_bounds__(256) void k_project_multi(ViewDev vd, SpectrumDev s
 }
 
 template <int M>
-static void launch_v1(bool atten, dim3 g, dim3 b, hipStream_t st, const Geo1 &geo, const VolDev &vol,
+static void launch_v1(bool atten, bool fast, dim3 g, dim3 b, hipStream_t st, const Geo1 &geo, const VolDev &vol,
                       const SpectrumDev &sp, float step, int air, const OutDev &o, int tx) {
-    if (atten)
-        k_project_v1<M, true><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
-    else
-        k_project_v1<M, false><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
+    if (fast) {
+        if (atten)
+            k_project_v1<M, true, true><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
+        else
+            k_project_v1<M, false, true><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
+    } else {
+        if (atten)
+            k_project_v1<M, true, false><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
+        else
+            k_project_v1<M, false, false><<<g, b, 0, st>>>(geo, vol, sp, step, air, o, tx);
+    }
 }
 
 template <int V>
@@ -172,7 +179,10 @@ extern "C" int drr_project(drr_ctx *ctx, const drr_volume *const *vols, const dr
     if (view->n_volumes == 1) {
         switch (sc.n_materials) {
 #define DRR_CASE(MM) \
-    case MM: launch_v1<MM>(sc.attenuate_outside != 0, grid, block, st, geo, vol0, sp, sc.step, sc.air_index, o, tx); break;
+    case MM:                                                                                           \
+        launch_v1<MM>(sc.attenuate_outside != 0, sc.march == DRR_MARCH_FAST, grid, block, st, geo, vol0, sp, \
+                      sc.step, sc.air_index, o, tx);                                                     \
+        break;
             DRR_CASE(1) DRR_CASE(2) DRR_CASE(3) DRR_CASE(4) DRR_CASE(5) DRR_CASE(6) DRR_CASE(7) DRR_CASE(8) \
         DRR_CASE(9) DRR_CASE(10) DRR_CASE(11) DRR_CASE(12) DRR_CASE(13) DRR_CASE(14) DRR_CASE(15) DRR_CASE(16)
 #undef DRR_CASE

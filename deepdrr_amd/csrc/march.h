@@ -23,6 +23,8 @@
 //     material; a fused multiply-add with a zero fraction leaves an
 //     accumulator unchanged, so the result is bit-identical to the
 //     per-material sums (451-455, 543-547).
+// The fast march (fast_run, DRR_MARCH_FAST) is the one place where values
+// differ from the reference's, by rounding only; it is described there.
 #pragma once
 
 #include "drr_internal.h"
@@ -290,6 +292,145 @@ __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad
     }
 }
 
+// The fast march (DRR_MARCH_FAST): the samples t in [t, tlim), all of weight 1,
+// with the same FP32 alpha sequence and the same sequential FP32 additions into
+// the area densities as sample_v1, but with each sample's density formed far
+// more cheaply than by the exact filter.  Along the ray the trilinear density
+// of a cell is a cubic in the distance u = alpha - a0 from the sample where the
+// ray entered the cell; on entering a cell its four coefficients are formed
+// from the 8 corner densities, the entry fractions and the ray direction, with
+// the distance to the exit face, and each sample then costs one subtraction, a
+// comparison with the exit distance and three multiply-adds.  The entry sample
+// uses the exact march's position and floor; the samples after it are placed
+// by the exact alpha sequence (u is exact) on the exact line instead of at
+// their rounded FP32 positions, and a sample within rounding of a face may
+// take either cell (the density is continuous there).  A cell whose labels
+// differ takes the exact filter's formulas.  Each sample's value thus differs
+// from the exact march's by a few roundings, with both signs, and the sums
+// stay within about 1e-7 relative of the exact march's; nothing depends on
+// the launch or on other lanes, so results are the same from run to run and
+// from kernel to kernel.
+struct FastCell {
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;  // density = ((c3 u + c2) u + c1) u + c0
+    float a0 = 0.f;    // alpha of the entry sample (u = alpha - a0)
+    float uex = -1.f;  // u at the exit face (< 0: no cell entered yet)
+    float qx = 0.f, qy = 0.f, qz = 0.f;  // b + 1 of the open cell
+
+    // Enter cell c at alpha with fractions (fx, fy, fz) (those of the exact
+    // march), the ray direction r in voxels per mm and its reciprocal ir.
+    __device__ __forceinline__ void enter(const Cell &c, float alpha, float fx, float fy, float fz, float rx,
+                                          float ry, float rz, float irx, float iry, float irz) {
+        a0 = alpha;
+        qx = c.bx + 1.0f;
+        qy = c.by + 1.0f;
+        qz = c.bz + 1.0f;
+        // x: e_yz(u) = A_yz + B_yz u, the lerp of corners (0yz, 1yz)
+        const float X00 = c.d0.y - c.d0.x, X10 = c.d0.w - c.d0.z, X01 = c.d1.y - c.d1.x, X11 = c.d1.w - c.d1.z;
+        const float A00 = __builtin_fmaf(fx, X00, c.d0.x), A10 = __builtin_fmaf(fx, X10, c.d0.z);
+        const float A01 = __builtin_fmaf(fx, X01, c.d1.x), A11 = __builtin_fmaf(fx, X11, c.d1.z);
+        const float B00 = rx * X00, B10 = rx * X10, B01 = rx * X01, B11 = rx * X11;
+        // y: g_z(u) = G_z0 + G_z1 u + G_z2 u^2
+        const float P0 = A10 - A00, Q0 = B10 - B00, P1 = A11 - A01, Q1 = B11 - B01;
+        const float G00 = __builtin_fmaf(fy, P0, A00), G01 = __builtin_fmaf(fy, Q0, __builtin_fmaf(ry, P0, B00));
+        const float G02 = ry * Q0;
+        const float G10 = __builtin_fmaf(fy, P1, A01), G11 = __builtin_fmaf(fy, Q1, __builtin_fmaf(ry, P1, B01));
+        const float G12 = ry * Q1;
+        // z: the cubic
+        const float D0 = G10 - G00, D1 = G11 - G01, D2 = G12 - G02;
+        c0 = __builtin_fmaf(fz, D0, G00);
+        c1 = __builtin_fmaf(fz, D1, __builtin_fmaf(rz, D0, G01));
+        c2 = __builtin_fmaf(fz, D2, __builtin_fmaf(rz, D1, G02));
+        c3 = rz * D2;
+        // exit: the first face a fraction reaches (1 going up, 0 going down;
+        // an axis the ray does not move along gives +inf)
+        const float ux = ((rx < 0.0f ? 0.0f : 1.0f) - fx) * irx;
+        const float uy = ((ry < 0.0f ? 0.0f : 1.0f) - fy) * iry;
+        const float uz = ((rz < 0.0f ? 0.0f : 1.0f) - fz) * irz;
+        uex = fminf(ux, fminf(uy, uz));
+    }
+};
+
+template <int M>
+__device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &fc, float (&ad)[M], Accum &a,
+                                         float sx, float sy, float sz, float rix, float riy, float riz,
+                                         float step, float &alpha, int &t, int tlim) {
+    // The loop runs with the whole wave (its test is a wave vote) and the
+    // common path applies per-lane choices as 0/1 factors: per-lane branches
+    // and loop exits cost scalar instructions (exec-mask bookkeeping), and
+    // this loop is short enough for those to become the limit; only the cell
+    // change and cells whose labels differ branch (and are skipped at the
+    // cost of one test when no lane needs them).  With g = 1,
+    // fma(g, rho, cur) = cur + rho and fma(g, step, alpha) = alpha + step
+    // exactly; with g = 0 both leave the value unchanged.
+    if (__ballot(t < tlim) == 0) return;
+    // reciprocals for the exit distances (1 ulp is plenty there; +-inf for an
+    // axis the ray does not move along)
+    const float irx = __builtin_amdgcn_rcpf(rix), iry = __builtin_amdgcn_rcpf(riy),
+                irz = __builtin_amdgcn_rcpf(riz);
+    float uf = c.uni ? 1.0f : 0.0f;  // 1 in a cell whose labels agree
+    do {
+        const bool live = t < tlim;
+        float u = alpha - fc.a0;
+        bool in = u < fc.uex;
+        // A lane whose next sample lies past its cell's exit waits until at
+        // least half of the lanes still marching wait too, and they enter their
+        // next cells together: the cell change is long and runs for the whole
+        // wave whenever any lane needs it, and for oblique rays some lane
+        // leaves its cell on most samples.  Waiting only delays a lane's next
+        // sample.
+        const bool leave = live & !in;
+        const bool now = 2 * __popcll(__ballot(leave)) >= __popcll(__ballot(live));
+        {
+            if (leave & now) {
+                const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
+                            pz = sample_pos(sz, alpha, riz);
+                const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+                c.load(vol, bx, by, bz);
+                fc.enter(c, alpha, px - bx, py - by, pz - bz, rix, riy, riz, irx, iry, irz);
+                u = 0.0f;
+                const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
+                if (ncm != a.cm) {
+                    if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
+                    a.cm = ncm;
+                    a.cur = get_m<M>(ad, ncm);
+                }
+                uf = c.uni ? 1.0f : 0.0f;
+                in = true;
+            }
+        }
+        const bool adv = live & in;
+        {
+            if (adv & (uf == 0.0f)) {
+                // (alpha through an empty asm: the fractions stay in this rare
+                // branch instead of being hoisted into every sample)
+                float al = alpha;
+                asm volatile("" : "+v"(al));
+                const float fx = __builtin_fmaf(al, rix, sx) - fc.qx;
+                const float fy = __builtin_fmaf(al, riy, sy) - fc.qy;
+                const float fz = __builtin_fmaf(al, riz, sz) - fc.qz;
+                float w[8];
+                weights8(fx, fy, fz, w);
+                const float x = trilerp(c.d0, c.d1, w);
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const uint32_t xm = c.l0 ^ (0x11111111u * (uint32_t)m);
+                    if (((xm - 0x11111111u) & ~xm & 0x88888888u) != 0u) {
+                        float sm = 0.0f;
+#pragma unroll
+                        for (int d = 0; d < 8; d++) sm += (corner_label(c.l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+                        ad[m] = __builtin_fmaf(x, sm, ad[m]);
+                    }
+                }
+            }
+        }
+        const float rho = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(fc.c3, u, fc.c2), u, fc.c1), u, fc.c0);
+        const float g = adv ? 1.0f : 0.0f;
+        a.cur = __builtin_fmaf(g * uf, rho, a.cur);
+        alpha = __builtin_fmaf(g, step, alpha);
+        t += adv ? 1 : 0;
+    } while (__ballot(t < tlim) != 0);
+}
+
 // Occupancy target of the march kernels: up to 5 materials the march fits in
 // 64 VGPRs (8 waves per SIMD) without spills; more materials get more registers.
 template <int M>
@@ -345,7 +486,7 @@ __device__ __forceinline__ void march_setup_v1(const Geo1 &g, int udx, int vdx, 
 
 // Single-volume march (project_kernel.cu:334-584 with NUM_VOLUMES == 1).
 // On return ad[] holds the final area densities in g/cm^2.
-template <int M, bool ATTEN>
+template <int M, bool ATTEN, bool FAST = false>
 __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int udx, int vdx,
                                          float step, int air, float (&ad)[M]) {
     RayV1 r;
@@ -412,12 +553,25 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
         // integers, where (float)(N - 1 - t) could round up)
         int tlim = min(t + (int)__builtin_amdgcn_fmed3f(kf, 0.0f, 16777216.0f), N - 1);
         for (;;) {
-            while (t < tlim) {
-                sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
-                             sample_pos(sz, alpha, riz), wt);
-                wt = 1.0f;
-                alpha += step;
-                ++t;
+            if (FAST) {
+                // a sample of weight 0.5 (the first) takes the exact filter
+                if (wt != 1.0f && t < tlim) {
+                    sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
+                                 sample_pos(sz, alpha, riz), wt);
+                    wt = 1.0f;
+                    alpha += step;
+                    ++t;
+                }
+                FastCell fc;  // no cell entered: the first sample enters one
+                fast_run<M>(vol, c, fc, ad, acc3, sx, sy, sz, rix, riy, riz, step, alpha, t, tlim);
+            } else {
+                while (t < tlim) {
+                    sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),
+                                 sample_pos(sz, alpha, riz), wt);
+                    wt = 1.0f;
+                    alpha += step;
+                    ++t;
+                }
             }
             // a fresh comparison after the loop (an empty asm hides alpha's
             // in-loop history), so the loop does not carry one

@@ -113,6 +113,8 @@ int drr_ctx_set_scene(drr_ctx *ctx, const drr_scene_t *scene) {
         return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: n_materials must be in [1, 16]");
     if (scene->air_index < 0 || scene->air_index >= scene->n_materials)
         return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: air_index out of range");
+    if (scene->march != DRR_MARCH_EXACT && scene->march != DRR_MARCH_FAST)
+        return drr_set_error(DRR_ERR_INVALID, "drr_ctx_set_scene: march must be DRR_MARCH_EXACT or DRR_MARCH_FAST");
     if (ctx->have_scene && ctx->n_bins > 0 && scene->n_materials != ctx->scene.n_materials) {
         ctx->n_bins = 0;  // mu table no longer matches; caller must set the spectrum again
     }

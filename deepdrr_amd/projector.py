@@ -23,7 +23,7 @@ from typing import List, Optional, Sequence, Union
 import numpy as np
 
 from . import _abi, geo, utils
-from ._abi_types import Scene, make_view
+from ._abi_types import MARCH_EXACT, MARCH_FAST, Scene, make_view
 from .material import Material, attenuation_table
 from .mesh import DRRMaterial, Mesh, mesh_arrays
 from .spectral_data import get_spectrum, spectrum_tables
@@ -111,14 +111,21 @@ class Projector:
         mesh_layers: int = 2,
         cuda_device_id: Optional[int] = None,
         mc_scatter_histories: int = 0,
+        march: str = "fast",
     ) -> None:
         """Arguments as deepdrr's ``Projector`` (projector.py:398-422), plus
         ``mc_scatter_histories``: photon histories per view of this engine's
         Monte-Carlo scatter (Woodcock tracking with the synthetic power-law
         cross sections of ``mc.SYNTHETIC_MU60``), added to the intensity
-        before noise and neglog; 0 (default) renders no scatter.  As in the
-        reference, ``scatter_num > 0`` (or ``add_scatter=True``) raises
-        ``DeprecationError``."""
+        before noise and neglog; 0 (default) renders no scatter; and
+        ``march``: ``"fast"`` (default) forms each sample's density from
+        per-cell cubics along the ray (images within 1e-5 relative of the
+        reference's filter, area densities within rounding of it) or
+        ``"exact"`` (the reference's per-sample filter and FP32 operation
+        order: area densities bit-identical to the oracle's transcription);
+        views over several volumes or with meshes are marched exactly in both.
+        As in the reference, ``scatter_num > 0`` (or ``add_scatter=True``)
+        raises ``DeprecationError``."""
         self.cuda_device_id = cuda_device_id
         self.mesh_layers = mesh_layers
         volume = utils.listify(volume)
@@ -177,6 +184,9 @@ class Projector:
         if mc_scatter_histories < 0:
             raise ValueError("mc_scatter_histories must be non-negative.")
         self.mc_scatter_histories = int(mc_scatter_histories)
+        if march not in ("fast", "exact"):
+            raise ValueError(f"march must be 'fast' or 'exact', not {march!r}")
+        self.march = march
         if self.mc_scatter_histories > 0 and self.device is None:
             raise ValueError("Must provide device to simulate scatter.")
 
@@ -274,7 +284,7 @@ class Projector:
         try:
             sc = Scene(step=self.step, n_materials=len(self.all_materials),
                        attenuate_outside=1 if self.attenuate_outside_volume else 0,
-                       air_index=self.air_index)
+                       air_index=self.air_index, march=MARCH_FAST if self.march == "fast" else MARCH_EXACT)
             _abi.call("drr_ctx_set_scene", self._ctx, C.byref(sc))
             self.energies, self.pdf = spectrum_tables(self.spectrum_arr)
             self.absorption_coef_table = attenuation_table(self.all_materials, self.energies)

@@ -61,12 +61,23 @@ typedef struct {
     drr_view_volume_t vol[DRR_MAX_VOLUMES];
 } drr_view_t;
 
+/* How a single-volume view is marched (drr_scene_t.march).  EXACT: the
+ * reference's per-sample trilinear filter with its FP32 operation order
+ * (area densities bit-identical to the oracle's transcription).  FAST: the same
+ * samples and the same sequential FP32 sums, with each sample's density formed
+ * by nested lerps from per-cell differences (values within rounding of EXACT,
+ * images within 1e-5 relative).  Views over several volumes or with meshes are
+ * marched exactly in both modes.  No counterpart in the reference. */
+#define DRR_MARCH_EXACT 0
+#define DRR_MARCH_FAST 1
+
 /* Scene-wide settings (the reference's compile-time macros and Projector kwargs). */
 typedef struct {
     float step;                /* mm along the unit world ray (projector.py:509) */
     int32_t n_materials;       /* NUM_MATERIALS */
     int32_t attenuate_outside; /* ATTENUATE_OUTSIDE_VOLUME */
     int32_t air_index;         /* AIR_INDEX */
+    int32_t march;             /* DRR_MARCH_EXACT (0) or DRR_MARCH_FAST */
 } drr_scene_t;
 
 /* ---- errors / context ---------------------------------------------------- */

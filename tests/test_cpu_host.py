@@ -136,6 +136,8 @@ def test_projector_constructor_errors():
         Projector(vol, max_mesh_hits=6)
     with pytest.raises(KeyError):
         Projector(vol, spectrum="100KV")
+    with pytest.raises(ValueError):
+        Projector(vol, march="approximate")
     # 16 materials are rendered (4-bit labels); a 17th is refused
     from deepdrr_amd.volume import Volume
 

@@ -1,0 +1,163 @@
+"""The fast march (DRR_MARCH_FAST) against the oracle's transcription.
+
+Bar (the fast march forms each sample's density by nested lerps instead of the
+exact 8-weight filter, so values differ from the transcription by rounding):
+intensity and photon_prob within 1e-5 relative of the transcription; every
+area density within 1e-5 of the pixel's total area density (the largest
+difference seen is printed); bit-identical from run to run; batched images
+bit-identical to one view per launch (the same march in every kernel).
+Scenes: the random volumes, poses, detectors and steps and the 1..16-material
+scenes of the exact tests, air outside on and off, and strided rows of the
+clinical 512x512x400 CT through the 1536^2 AP view and batch poses.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from deepdrr_amd._abi_types import MARCH_FAST
+from tests import scenes
+from tests.scenes import GpuRunner
+from tests.test_gpu_forward import _materials_scene, _random_scene
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def _fast(case):
+    case.scene.march = MARCH_FAST
+    return case
+
+
+def _check_close(name, gpu_out, ora_out):
+    """1e-5 bars; returns the largest area-density difference relative to the
+    pixel's total."""
+    I, P, AD = gpu_out[:3]
+    Io, Po, ADo = ora_out
+    assert np.all(np.isfinite(I)) and np.all(np.isfinite(AD))
+    np.testing.assert_allclose(I, Io, rtol=REL, atol=0, err_msg=name)
+    np.testing.assert_allclose(P, Po, rtol=REL, atol=0, err_msg=name)
+    tot = ADo.astype(np.float64).sum(0)
+    diff = np.abs(AD.astype(np.float64) - ADo).max(0)
+    rel = np.where(tot > 0, diff / np.where(tot > 0, tot, 1.0), diff)
+    worst = float(rel.max()) if rel.size else 0.0
+    assert worst <= REL, f"{name}: area density off by {worst:.3g} of the pixel total"
+    return worst
+
+
+def _oracle(case, rows=None):
+    from oracle import ctrans
+
+    return ctrans.project(case.volumes, case.view, case.scene, case.E, case.pdf, case.mu, rows=rows)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_fast_random_scenes(gpu, seed):
+    case = _fast(_random_scene(seed))
+    print(f"case {case.name} (fast)", flush=True)
+    r = GpuRunner(case)
+    try:
+        g1 = r.project()
+        g2 = r.project()
+        for a, b in zip(g1, g2):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), "not bit-stable across runs"
+        w = _check_close(case.name, g1, _oracle(case))
+        print(f"{case.name}: largest area-density difference {w:.3g} of the pixel total")
+    finally:
+        r.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 11, 16])
+def test_fast_material_counts(gpu, k):
+    """Uniform and mixed cells with 1..16 materials; batched images (both tile
+    shapes) bit-identical to one view per launch."""
+    import torch
+
+    from deepdrr_amd.batch import BatchPlan
+
+    case, vol, projs = _materials_scene(40 + k, k)
+    _fast(case)
+    print(f"case {case.name} (fast)", flush=True)
+    r = GpuRunner(case)
+    try:
+        w = _check_close(case.name, r.project(), _oracle(case))
+        print(f"{case.name}: largest area-density difference {w:.3g} of the pixel total")
+        views = [case.view] + [scenes.P.build_view([vol], [0], pr, 4080.0) for pr in projs[1:]]
+        singles = [r.project(v)[:2] for v in views]
+        for tuning in ((8, 8, 4, 8), (16, 4, 2, 4)):
+            plan = BatchPlan(r.ctx, [r.vols[0]], views, [0] * len(views), neglog=False, tuning=tuning)
+            plan.launch()
+            torch.cuda.synchronize()
+            I_b, P_b = plan.fetch("intensity"), plan.fetch("photon_prob")
+            plan.close()
+            for i, (I, P) in enumerate(singles):
+                assert np.array_equal(I.view(np.uint32), I_b[i].view(np.uint32)), f"view {i}, {tuning}"
+                assert np.array_equal(P.view(np.uint32), P_b[i].view(np.uint32)), f"view {i}, {tuning}"
+    finally:
+        r.close()
+
+
+def test_fast_clinical_rows(gpu):
+    """The clinical CT (rays of ~4000 samples): 24 strided rows of the 1536^2
+    AP view and 4 of each of two batch poses against the transcription, with
+    the largest FP32-vs-FP64 error of the same pixels printed beside the
+    transcription's; the batch of the three views bit-identical to single
+    views."""
+    import torch
+
+    from oracle import fp64
+    from deepdrr_amd import phantoms
+    from deepdrr_amd.batch import BatchPlan
+
+    vol = phantoms.clinical_ct()
+    ap = phantoms.ap_device(1536, 1536, 0.194).get_camera_projection()
+    poses = phantoms.batch_poses(2, seed=0)
+    projs = [ap] + [phantoms.pose_projection(1536, 1536, 0.194, p, d, f) for p, d, f in poses]
+    case = _fast(scenes.make_case("clinical_fast", [vol], ap))
+    print("case clinical_fast", flush=True)
+    r = GpuRunner(case)
+    try:
+        views = [scenes.P.build_view([vol], [0], pr, 4080.0) for pr in projs]
+        outs = [r.project(v) for v in views]
+        for i, v in enumerate(views):
+            rows = np.linspace(0, 1535, 24 if i == 0 else 4).astype(np.int32)
+            case.view = v
+            Io, Po, ADo = _oracle(case, rows=rows)
+            I, P, AD = outs[i]
+            w = _check_close(f"clinical view {i}", (I[rows], P[rows], AD[:, rows]), (Io, Po, ADo))
+            if i == 0:
+                cols = np.arange(16, 1536, 64)
+                pix = np.array([rr * 1536 + cc for rr in rows[::2] for cc in cols])
+                I64, P64, _ = fp64.project_fp64(case.volumes, case.view_dict(), case.scene_dict(), case.E,
+                                                case.pdf, case.mu, pixels=pix)
+                sel = np.isin(np.arange(24), np.arange(0, 24, 2))
+                Ir = I[rows[sel]][:, cols].reshape(-1)
+                Ior = Io[sel][:, cols].reshape(-1)
+                print(f"clinical AP: fast vs transcription area density {w:.3g} of the total, intensity "
+                      f"{np.max(np.abs(I[rows] - Io) / Io):.3g}; vs FP64 fast {np.max(np.abs(Ir - I64) / I64):.3g}, "
+                      f"transcription {np.max(np.abs(Ior - I64) / I64):.3g}")
+        plan = BatchPlan(r.ctx, [r.vols[0]], views, [0] * len(views), neglog=False, tuning=(8, 8, 4, 8))
+        plan.launch()
+        torch.cuda.synchronize()
+        I_b = plan.fetch("intensity")
+        plan.close()
+        for i, o in enumerate(outs):
+            assert np.array_equal(o[0].view(np.uint32), I_b[i].view(np.uint32)), f"view {i}"
+    finally:
+        r.close()
+
+
+def test_fast_scene_flag_checked(gpu):
+    """march must be DRR_MARCH_EXACT or DRR_MARCH_FAST."""
+    from deepdrr_amd import _abi
+    from deepdrr_amd._abi_types import Scene
+
+    ctx = C.c_void_p()
+    _abi.call("drr_ctx_create", 0, C.byref(ctx))
+    try:
+        sc = Scene(step=0.1, n_materials=3, attenuate_outside=0, air_index=0, march=7)
+        assert _abi.lib().drr_ctx_set_scene(ctx, C.byref(sc)) != 0
+        assert "march" in _abi.last_error()
+    finally:
+        _abi.lib().drr_ctx_destroy(ctx)

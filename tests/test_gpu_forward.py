@@ -142,21 +142,29 @@ def test_disabled_volume(gpu):
 
 def test_projector_dropin(gpu):
     """Projector(volume, device).initialize()/__call__ returns the neglog image the
-    reference pipeline computes from the oracle's intensities."""
+    reference pipeline computes from the oracle's intensities, with both
+    marches (exact: area densities bit-identical; fast: within 1e-5)."""
     from deepdrr_amd import Projector, phantoms, utils
     from deepdrr_amd.spectral_data import synthetic_120kv
 
     vol = phantoms.small_ct(32)
     dev = phantoms.ap_device(40, 30, 1.6)
-    with Projector(vol, device=dev, step=0.1, spectrum=synthetic_120kv()) as proj:
-        img = proj()
-        assert img.shape == (30, 40) and img.dtype in (np.float32, np.float64)
-        AD, I, P = proj.project_area_density(dev.get_camera_projection())
-        case = scenes.make_case("dropin", [vol], dev.get_camera_projection(),
-                                max_ray_length=proj.max_ray_length)
+    out = {}
+    for march in ("exact", "fast"):
+        with Projector(vol, device=dev, step=0.1, spectrum=synthetic_120kv(), march=march) as proj:
+            img = proj()
+            assert img.shape == (30, 40) and img.dtype in (np.float32, np.float64)
+            out[march] = (img,) + proj.project_area_density(dev.get_camera_projection())
+            case = scenes.make_case("dropin", [vol], dev.get_camera_projection(),
+                                    max_ray_length=proj.max_ray_length)
     Io, Po, ADo = _oracle(case)
+    img, AD, I, P = out["exact"]
     np.testing.assert_allclose(utils.neglog(Io[None])[0], img, atol=1e-5)
     assert np.array_equal(AD.view(np.uint32), ADo.view(np.uint32))
+    img, AD, I, P = out["fast"]
+    np.testing.assert_allclose(utils.neglog(Io[None])[0], img, atol=1e-5)
+    np.testing.assert_allclose(I, Io, rtol=1e-5)
+    assert np.all(np.abs(AD - ADo) <= 1e-5 * ADo.sum(0))
 
 
 def test_projector_multiview_and_errors(gpu):
