@@ -61,37 +61,45 @@ def make_ctx(_abi, materials, step, march="fast"):
 PMC_SUMMARY = os.path.join("profiles", "k_batch_march_clinical_1536_pmc.md")
 
 
-def profiled_counters(views_per_launch):
+def profiled_counters(views_per_launch, march, tuning):
     """Counter figures of the batched march on this workload, read from the
-    committed rocprofv3 summary: HBM bytes per launch (FETCH_SIZE + WRITE_SIZE,
-    each scaled by a copy-kernel calibration), the VALU issue figures
-    (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2, SQ_CYCLES) and the L2 hit rate
-    (TCC_HIT / (TCC_HIT + TCC_MISS)).  Returns (bytes, schedule the counters were
-    taken with, valu dict, L2 hit rate), Nones when the summary is absent or was
-    taken at another batch size."""
+    committed rocprofv3 summary, for the record of this march and schedule:
+    HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, each scaled by a copy-kernel
+    calibration), the VALU issue figures (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2,
+    SQ_CYCLES) and the L2 hit rate (TCC_HIT / (TCC_HIT + TCC_MISS)).  Returns
+    (bytes, valu dict, L2 hit rate), Nones when the summary holds no record
+    for this batch size, march and schedule."""
     path = os.path.join(ROOT, PMC_SUMMARY)
     if not os.path.exists(path):
-        return None, None, None, None
-    vals = {}
-    keys = ("traffic_bytes_per_launch", "views_per_launch", "tuning", "valu_issue_frac",
+        return None, None, None
+    keys = ("march", "tuning", "views_per_launch", "traffic_bytes_per_launch", "valu_issue_frac",
             "valu_dual_issue_share", "valu_lane_instructions_per_sample", "l2_hit_rate")
+    records, cur = [], {}
     for ln in open(path):
         ln = ln.strip()
+        if ln.startswith("march:") and cur:
+            records.append(cur)
+            cur = {}
         for key in keys:
             if ln.startswith(key + ":"):
-                vals[key] = ln.split(":", 1)[1].strip()
-    if "traffic_bytes_per_launch" not in vals or int(vals.get("views_per_launch", -1)) != views_per_launch:
-        return None, None, None, None
-    valu = None
-    if "valu_issue_frac" in vals:
-        valu = {"issue_frac": float(vals["valu_issue_frac"]),
-                "dual_issue_share": float(vals["valu_dual_issue_share"]),
-                "lane_instructions_per_sample": float(vals["valu_lane_instructions_per_sample"]),
-                "note": "VALU issue quad-cycles used / available over the launch "
-                        "((SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2) / (SQ_CYCLES x 8)); the kernel's binding "
-                        "resource", "source": PMC_SUMMARY}
-    l2 = float(vals["l2_hit_rate"]) if "l2_hit_rate" in vals else None
-    return float(vals["traffic_bytes_per_launch"]), [int(x) for x in vals.get("tuning", "").split()], valu, l2
+                cur[key] = ln.split(":", 1)[1].strip()
+    if cur:
+        records.append(cur)
+    want = " ".join(str(int(x)) for x in tuning)
+    for vals in records:
+        if (vals.get("march") == march and vals.get("tuning") == want
+                and int(vals.get("views_per_launch", -1)) == views_per_launch and "traffic_bytes_per_launch" in vals):
+            valu = None
+            if "valu_issue_frac" in vals:
+                valu = {"issue_frac": float(vals["valu_issue_frac"]),
+                        "dual_issue_share": float(vals["valu_dual_issue_share"]),
+                        "lane_instructions_per_sample": float(vals["valu_lane_instructions_per_sample"]),
+                        "note": "VALU issue quad-cycles used / available over the launch "
+                                "((SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2) / (SQ_CYCLES x 8)); the kernel's "
+                                "binding resource", "source": PMC_SUMMARY}
+            l2 = float(vals["l2_hit_rate"]) if "l2_hit_rate" in vals else None
+            return float(vals["traffic_bytes_per_launch"]), valu, l2
+    return None, None, None
 
 
 def plain_views(torch, _abi, ctx, varr, views, dev, utils):
@@ -395,7 +403,7 @@ def main():
     from deepdrr_amd import autotune
 
     tune_key = autotune.workload_key(views, [0] * B, [(512, 512, 400)], len(materials),
-                                     torch.cuda.get_device_name(0))
+                                     torch.cuda.get_device_name(0), march=args.march)
     t = time.perf_counter()
     plan = BatchPlan(ctx, [vh], views, [0] * B, neglog=True,
                      tuning=tuple(args.tuning) if args.tuning else "auto", tune_key=tune_key)
@@ -434,7 +442,7 @@ def main():
     other_march_ms = float(np.mean(kms_o))
     log(f"{other} march of the same batch: {other_march_ms / B:.2f} ms/view")
     gather_gbs = samples_per_launch * workload.BYTES_PER_SAMPLE / (kernel_ms * 1e-3) / 1e9
-    traffic, traffic_tuning, valu, l2_hit = profiled_counters(B)
+    traffic, valu, l2_hit = profiled_counters(B, args.march, plan.tuning)
     # HBM bytes the launch moves (rocprof FETCH_SIZE + WRITE_SIZE of the same
     # launch, copy-calibrated) over the live kernel time
     achieved = traffic / (kernel_ms * 1e-3) / 1e9 if traffic is not None else None
@@ -460,10 +468,11 @@ def main():
         roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK_GBS if achieved is not None else None, "traffic": traffic,
                   "achieved_source": "traffic (HBM bytes per launch, rocprofv3) / kernel_ms (HIP events, live)",
-                  "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated, schedule "
-                                     f"{traffic_tuning} ({PMC_SUMMARY})" if traffic is not None else None),
+                  "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated, the same march and "
+                                     f"schedule ({PMC_SUMMARY})" if traffic is not None else None),
                   "valu": valu, "l2_hit_rate": l2_hit,
-                  "kernel": f"k_batch_march<{len(materials)},false,{plan.tuning[0]}>",
+                  "kernel": f"k_batch_march<{len(materials)},false,{plan.tuning[0]},"
+                            f"{'true' if args.march == 'fast' else 'false'}>",
                   "kernel_ms": kernel_ms, "samples_per_launch": samples_per_launch,
                   "samples_per_s": samples_per_launch / (kernel_ms * 1e-3),
                   "gather_equivalent": {

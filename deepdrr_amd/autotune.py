@@ -44,14 +44,15 @@ def valid(t: Sequence[int]) -> bool:
 
 
 def workload_key(views: Sequence, volume_indices: Sequence[int], vol_shapes: Sequence[Tuple[int, int, int]],
-                 n_materials: int, device_name: str = "") -> str:
-    """Detector sizes, volume shapes and materials; not the poses (the best
-    schedule does not depend on them to first order)."""
+                 n_materials: int, device_name: str = "", march: str = "") -> str:
+    """Detector sizes, volume shapes, materials and the march ("fast" or
+    "exact": their loops differ, and so do their best schedules); not the
+    poses (the best schedule does not depend on them to first order)."""
     dets = sorted({(int(v.width), int(v.height)) for v in views})
     vols = sorted({tuple(int(s) for s in vol_shapes[i]) for i in set(int(x) for x in volume_indices)})
     nb = 1 << max(0, (len(views) - 1).bit_length())  # batch size bucket
-    return json.dumps({"dev": device_name, "M": int(n_materials), "det": dets, "vol": vols, "n": nb},
-                      sort_keys=True)
+    return json.dumps({"dev": device_name, "M": int(n_materials), "det": dets, "vol": vols, "n": nb,
+                       "march": march}, sort_keys=True)
 
 
 def _load_file():

@@ -149,7 +149,7 @@ def plan_for_projector(projector, projections: Sequence[geo.CameraProjection],
         from . import autotune
 
         key = autotune.workload_key(views, volume_indices, [v.shape for v in projector.volumes],
-                                    len(projector.all_materials))
+                                    len(projector.all_materials), march=projector.march)
     return BatchPlan(projector._ctx, projector._vols, views, volume_indices,
                      neglog=projector.neglog if neglog is None else neglog, tuning=tuning,
                      device=projector._torch_device, tune_key=key)

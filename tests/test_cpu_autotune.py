@@ -23,8 +23,9 @@ def test_workload_key_ignores_poses_not_sizes():
     b = autotune.workload_key([_view(64, 64)] * 6, [0] * 6, [(10, 11, 12)], 3, "gfx950")
     c = autotune.workload_key([_view(64, 32)] * 5, [0] * 5, [(10, 11, 12)], 3, "gfx950")
     d = autotune.workload_key([_view(64, 64)] * 5, [0] * 5, [(10, 11, 12)], 5, "gfx950")
+    e = autotune.workload_key([_view(64, 64)] * 5, [0] * 5, [(10, 11, 12)], 3, "gfx950", march="fast")
     assert a == b  # same size bucket (5 and 6 -> 8)
-    assert a != c and a != d
+    assert a != c and a != d and a != e
     assert json.loads(a)["det"] == [[64, 64]]
 
 
