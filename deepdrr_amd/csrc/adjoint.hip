@@ -56,6 +56,7 @@ struct CellCorners {
 
 // Straight to the gradient volume with float atomics.
 struct GlobalDep {
+    static constexpr bool kDensity = true;
     float *gD;
     // some of the 8 corners (the others are zero in v and skipped per lane)
     __device__ __forceinline__ void part(const VolDev &vol, float bx, float by, float bz, const float (&v)[8]) {
@@ -79,6 +80,7 @@ struct GlobalDep {
 
 // Nowhere: the pose gradient alone (the density is not differentiated).
 struct NullDep {
+    static constexpr bool kDensity = false;
     __device__ __forceinline__ void operator()(const VolDev &, float, float, float, float (&acc)[8]) {
 #pragma unroll
         for (int d = 0; d < 8; d++) acc[d] = 0.0f;
@@ -89,6 +91,7 @@ struct NullDep {
 // Into a workgroup's LDS box [o, o + L) (voxels, per axis); corners outside it
 // go to the gradient volume directly, so the box only changes the speed.
 struct BoxDep {
+    static constexpr bool kDensity = true;
     float *box;
     int ox, oy, oz, Lx, Ly, Lz;
     float *gD;
@@ -173,6 +176,10 @@ struct AdjRay {
     float graw[M];
     float acc[8];   // sum over the cell's samples of G w_d: density gradient of corner d
     float acc1[8];  // sum of alpha G w_d (pose only)
+    // (pose kernel) density sums carried into the open cell through the face
+    // it was entered by, held apart from acc so the pose term sees the cell's
+    // own sums
+    float carry[8];
     float hu;       // dL/d(ad_raw) of the material of a cell whose labels agree
     // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t alpha_t dL/dp_t, in FP64
     // (thousands of terms of both signs), kept in LDS (S[k * 256] for k < 6,
@@ -211,6 +218,7 @@ struct AdjRay {
         for (int d = 0; d < 8; d++) {
             acc[d] = 0.0f;
             acc1[d] = 0.0f;
+            carry[d] = 0.0f;
         }
         hu = 0.0f;
         if (POSE)
@@ -265,24 +273,36 @@ struct AdjRay {
     template <class Dep>
     __device__ __forceinline__ void flush(const VolDev &vol, Dep &dep) {
         if (!c.have) return;
-        if (POSE) flush_pose();
+        if (POSE) {
+            flush_pose();
+#pragma unroll
+            for (int d = 0; d < 8; d++) {
+                acc[d] += carry[d];
+                carry[d] = 0.0f;
+            }
+        }
         dep(vol, c.bx, c.by, c.bz, acc);
     }
 
-    // (Density only.) The ray leaves the open cell through its face normal to
-    // axis k (s = +1 towards +k, -1 towards -k), into the neighbouring cell.
-    // The 4 corners on that face are corners of the next cell too: their sums
-    // stay in registers (moved to the next cell's corner slots) and only the 4
-    // corners left behind are handed to `dep`, lane by lane, so a face
-    // crossing costs half the LDS atomic lanes of a full flush.  (With the pose
-    // sums, a cell's pose term needs that cell's own sums, so the pose kernel
-    // flushes all 8 corners.)
+    // The ray leaves the open cell through its face normal to axis k (s = +1
+    // towards +k, -1 towards -k), into the neighbouring cell.  The 4 corners
+    // on that face are corners of the next cell too: their sums stay in
+    // registers (moved to the next cell's corner slots) and only the 4 corners
+    // left behind are handed to `dep`, lane by lane, so a face crossing costs
+    // half the LDS atomic lanes of a full flush.  The density-only kernel
+    // carries them in acc itself; the pose kernel, whose pose term needs the
+    // cell's own sums, flushes its pose term first and carries them in carry.
     template <class Dep>
     __device__ __forceinline__ void leave_through_face(const VolDev &vol, Dep &dep, int k, int s) {
         const int f = s < 0 ? 1 : 0;  // bit along k of the corners left behind
         // (masks and products instead of selects between array elements, which
         // the compiler would turn into indexed loads from scratch memory)
         const float m0 = k == 0 ? 1.0f : 0.0f, m1 = k == 1 ? 1.0f : 0.0f, m2 = k == 2 ? 1.0f : 0.0f;
+        if (POSE) {
+            flush_pose();
+#pragma unroll
+            for (int d = 0; d < 8; d++) acc[d] += carry[d];
+        }
         float leave[8], nacc[8];
 #pragma unroll
         for (int d = 0; d < 8; d++) {
@@ -294,7 +314,14 @@ struct AdjRay {
         }
         dep.part(vol, c.bx, c.by, c.bz, leave);
 #pragma unroll
-        for (int d = 0; d < 8; d++) acc[d] = nacc[d];
+        for (int d = 0; d < 8; d++) {
+            if (POSE) {
+                carry[d] = nacc[d];
+                acc[d] = 0.0f;
+            } else {
+                acc[d] = nacc[d];
+            }
+        }
     }
 
     // March on while the floor of the sample position along `axis` of each
@@ -333,10 +360,9 @@ struct AdjRay {
             if (!same && change_now) {
                 const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
                 // one step to a face neighbour (in clamped cell coordinates): carry
-                // the shared face (density-only kernel); anything else: hand over
-                // all 8 corners
+                // the shared face; anything else: hand over all 8 corners
                 int dd[3] = {0, 0, 0};
-                if (!POSE && c.have) {
+                if (Dep::kDensity && c.have) {
                     dd[0] = clamp_cell(bx, vol.nx) - clamp_cell(c.bx, vol.nx);
                     dd[1] = clamp_cell(by, vol.ny) - clamp_cell(c.by, vol.ny);
                     dd[2] = clamp_cell(bz, vol.nz) - clamp_cell(c.bz, vol.nz);
