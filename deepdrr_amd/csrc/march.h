@@ -313,17 +313,13 @@ __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad
 struct FastCell {
     float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;  // density = ((c3 u + c2) u + c1) u + c0
     float a0 = 0.f;    // alpha of the entry sample (u = alpha - a0)
-    float uex = -1.f;  // u at the exit face (< 0: no cell entered yet)
-    float qx = 0.f, qy = 0.f, qz = 0.f;  // b + 1 of the open cell
+    float uex = -1.f;  // u at the exit face
 
     // Enter cell c at alpha with fractions (fx, fy, fz) (those of the exact
     // march), the ray direction r in voxels per mm and its reciprocal ir.
     __device__ __forceinline__ void enter(const Cell &c, float alpha, float fx, float fy, float fz, float rx,
                                           float ry, float rz, float irx, float iry, float irz) {
         a0 = alpha;
-        qx = c.bx + 1.0f;
-        qy = c.by + 1.0f;
-        qz = c.bz + 1.0f;
         // x: e_yz(u) = A_yz + B_yz u, the lerp of corners (0yz, 1yz)
         const float X00 = c.d0.y - c.d0.x, X10 = c.d0.w - c.d0.z, X01 = c.d1.y - c.d1.x, X11 = c.d1.w - c.d1.z;
         const float A00 = __builtin_fmaf(fx, X00, c.d0.x), A10 = __builtin_fmaf(fx, X10, c.d0.z);
@@ -354,80 +350,74 @@ template <int M>
 __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &fc, float (&ad)[M], Accum &a,
                                          float sx, float sy, float sz, float rix, float riy, float riz,
                                          float step, float &alpha, int &t, int tlim) {
-    // The loop runs with the whole wave (its test is a wave vote) and the
-    // common path applies per-lane choices as 0/1 factors: per-lane branches
-    // and loop exits cost scalar instructions (exec-mask bookkeeping), and
-    // this loop is short enough for those to become the limit; only the cell
-    // change and cells whose labels differ branch (and are skipped at the
-    // cost of one test when no lane needs them).  With g = 1,
-    // fma(g, rho, cur) = cur + rho and fma(g, step, alpha) = alpha + step
-    // exactly; with g = 0 both leave the value unchanged.
+    // Cell by cell: every lane still marching enters the cell of its current
+    // sample together (one execution of the long cell-entry block for the whole
+    // wave), then the wave runs the samples of those cells until every lane has
+    // left its own.  Lanes whose cells end early idle for the rest of that inner
+    // loop, which costs less than running the entry block once for each group
+    // of lanes that leave together.  The inner loop's test is a wave vote and
+    // its per-lane choices are 0/1 factors: per-lane branches and loop exits cost
+    // scalar instructions (exec-mask bookkeeping), enough to become the limit in
+    // a loop this short.  With g = 1, fma(g, rho, cur) = cur + rho and
+    // fma(g, step, alpha) = alpha + step exactly; with g = 0 both leave the
+    // value unchanged.
     if (__ballot(t < tlim) == 0) return;
     // reciprocals for the exit distances (1 ulp is plenty there; +-inf for an
     // axis the ray does not move along)
     const float irx = __builtin_amdgcn_rcpf(rix), iry = __builtin_amdgcn_rcpf(riy),
                 irz = __builtin_amdgcn_rcpf(riz);
-    float uf = c.uni ? 1.0f : 0.0f;  // 1 in a cell whose labels agree
     do {
         const bool live = t < tlim;
-        float u = alpha - fc.a0;
-        bool in = u < fc.uex;
-        // A lane whose next sample lies past its cell's exit waits until at
-        // least half of the lanes still marching wait too, and they enter their
-        // next cells together: the cell change is long and runs for the whole
-        // wave whenever any lane needs it, and for oblique rays some lane
-        // leaves its cell on most samples.  Waiting only delays a lane's next
-        // sample.
-        const bool leave = live & !in;
-        const bool now = 2 * __popcll(__ballot(leave)) >= __popcll(__ballot(live));
-        {
-            if (leave & now) {
-                const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
-                            pz = sample_pos(sz, alpha, riz);
-                const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
-                c.load(vol, bx, by, bz);
-                fc.enter(c, alpha, px - bx, py - by, pz - bz, rix, riy, riz, irx, iry, irz);
-                u = 0.0f;
-                const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
-                if (ncm != a.cm) {
-                    if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
-                    a.cm = ncm;
-                    a.cur = get_m<M>(ad, ncm);
-                }
-                uf = c.uni ? 1.0f : 0.0f;
-                in = true;
+        float uf = 0.0f;  // 1 in a cell whose labels agree
+        if (live) {
+            const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
+                        pz = sample_pos(sz, alpha, riz);
+            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+            c.load(vol, bx, by, bz);
+            fc.enter(c, alpha, px - bx, py - by, pz - bz, rix, riy, riz, irx, iry, irz);
+            const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
+            if (ncm != a.cm) {
+                if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
+                a.cm = ncm;
+                a.cur = get_m<M>(ad, ncm);
             }
+            uf = c.uni ? 1.0f : 0.0f;
         }
-        const bool adv = live & in;
-        {
-            if (adv & (uf == 0.0f)) {
-                // (alpha through an empty asm: the fractions stay in this rare
-                // branch instead of being hoisted into every sample)
+        bool in = live;  // the entry sample is always taken
+        float u = 0.0f;
+        do {
+            if (in & (uf == 0.0f)) {
+                // (alpha and the labels through an empty asm: the fractions and
+                // the label tests stay in this rare branch instead of being
+                // hoisted into every sample or every cell entry)
                 float al = alpha;
-                asm volatile("" : "+v"(al));
-                const float fx = __builtin_fmaf(al, rix, sx) - fc.qx;
-                const float fy = __builtin_fmaf(al, riy, sy) - fc.qy;
-                const float fz = __builtin_fmaf(al, riz, sz) - fc.qz;
+                uint32_t l0 = c.l0;
+                asm volatile("" : "+v"(al), "+v"(l0));
+                const float fx = __builtin_fmaf(al, rix, sx) - (c.bx + 1.0f);
+                const float fy = __builtin_fmaf(al, riy, sy) - (c.by + 1.0f);
+                const float fz = __builtin_fmaf(al, riz, sz) - (c.bz + 1.0f);
                 float w[8];
                 weights8(fx, fy, fz, w);
                 const float x = trilerp(c.d0, c.d1, w);
 #pragma unroll
                 for (int m = 0; m < M; m++) {
-                    const uint32_t xm = c.l0 ^ (0x11111111u * (uint32_t)m);
+                    const uint32_t xm = l0 ^ (0x11111111u * (uint32_t)m);
                     if (((xm - 0x11111111u) & ~xm & 0x88888888u) != 0u) {
                         float sm = 0.0f;
 #pragma unroll
-                        for (int d = 0; d < 8; d++) sm += (corner_label(c.l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+                        for (int d = 0; d < 8; d++) sm += (corner_label(l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
                         ad[m] = __builtin_fmaf(x, sm, ad[m]);
                     }
                 }
             }
-        }
-        const float rho = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(fc.c3, u, fc.c2), u, fc.c1), u, fc.c0);
-        const float g = adv ? 1.0f : 0.0f;
-        a.cur = __builtin_fmaf(g * uf, rho, a.cur);
-        alpha = __builtin_fmaf(g, step, alpha);
-        t += adv ? 1 : 0;
+            const float rho = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(fc.c3, u, fc.c2), u, fc.c1), u, fc.c0);
+            const float g = in ? 1.0f : 0.0f;
+            a.cur = __builtin_fmaf(g * uf, rho, a.cur);
+            alpha = __builtin_fmaf(g, step, alpha);
+            t += in ? 1 : 0;
+            u = alpha - fc.a0;
+            in = in & (t < tlim) & (u < fc.uex);
+        } while (__ballot(in) != 0);
     } while (__ballot(t < tlim) != 0);
 }
 
