@@ -353,9 +353,13 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     // Cell by cell: every lane still marching enters the cell of its current
     // sample together (one execution of the long cell-entry block for the whole
     // wave), then the wave runs the samples of those cells until every lane has
-    // left its own.  Lanes whose cells end early idle for the rest of that inner
+    // taken its own.  Lanes whose cells end early idle for the rest of that inner
     // loop, which costs less than running the entry block once for each group
-    // of lanes that leave together.  The inner loop's test is a wave vote and
+    // of lanes that leave together.  A cell's sample count n is fixed on entry:
+    // the samples at u = 0, step, 2 step, ... before the exit distance (at least
+    // the entry sample, at most the samples left before tlim); a sample within
+    // the FP32 alpha sequence's drift of the face may so fall to either cell,
+    // where the density is continuous.  The inner loop's test is a wave vote and
     // its per-lane choices are 0/1 factors: per-lane branches and loop exits cost
     // scalar instructions (exec-mask bookkeeping), enough to become the limit in
     // a loop this short.  With g = 1, fma(g, rho, cur) = cur + rho and
@@ -363,13 +367,15 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     // value unchanged.
     if (__ballot(t < tlim) == 0) return;
     // reciprocals for the exit distances (1 ulp is plenty there; +-inf for an
-    // axis the ray does not move along)
-    const float irx = __builtin_amdgcn_rcpf(rix), iry = __builtin_amdgcn_rcpf(riy),
-                irz = __builtin_amdgcn_rcpf(riz);
+    // axis the ray does not move along), kept in registers (an empty asm stops
+    // the compiler from forming them again on every cell entry)
+    float irx = __builtin_amdgcn_rcpf(rix), iry = __builtin_amdgcn_rcpf(riy), irz = __builtin_amdgcn_rcpf(riz);
+    asm volatile("" : "+v"(irx), "+v"(iry), "+v"(irz));
+    const float istep = 1.0f / step;
     do {
-        const bool live = t < tlim;
+        int n = 0;        // samples of the entered cell
         float uf = 0.0f;  // 1 in a cell whose labels agree
-        if (live) {
+        if (t < tlim) {
             const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
                         pz = sample_pos(sz, alpha, riz);
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
@@ -382,10 +388,15 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
                 a.cur = get_m<M>(ad, ncm);
             }
             uf = c.uni ? 1.0f : 0.0f;
+            // ceil(uex / step), clamped to [1, tlim - t] after the conversion
+            // (which saturates infinities and takes NaN to 0), so every entry
+            // takes at least one sample
+            n = min(max((int)ceilf(fc.uex * istep), 1), tlim - t);
         }
-        bool in = live;  // the entry sample is always taken
         float u = 0.0f;
-        do {
+        for (int k = 0;; ++k) {
+            const bool in = k < n;
+            if (__ballot(in) == 0) break;
             if (in & (uf == 0.0f)) {
                 // (alpha and the labels through an empty asm: the fractions and
                 // the label tests stay in this rare branch instead of being
@@ -414,10 +425,9 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
             const float g = in ? 1.0f : 0.0f;
             a.cur = __builtin_fmaf(g * uf, rho, a.cur);
             alpha = __builtin_fmaf(g, step, alpha);
-            t += in ? 1 : 0;
             u = alpha - fc.a0;
-            in = in & (t < tlim) & (u < fc.uex);
-        } while (__ballot(in) != 0);
+        }
+        t += n;
     } while (__ballot(t < tlim) != 0);
 }
 
