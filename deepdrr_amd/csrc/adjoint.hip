@@ -325,40 +325,34 @@ struct AdjRay {
     }
 
     // March on while the floor of the sample position along `axis` of each
-    // newly entered cell satisfies go(floor).  The open cell's
-    // sums are flushed when the ray leaves it or ends.
+    // newly entered cell satisfies go(floor).  The open cell's sums are handed
+    // over when the ray leaves it or ends.
+    //
+    // Cell by cell: every lane still marching hands over its open cell and
+    // loads the cell of its current sample together (one execution of that
+    // long block for the whole wave), then the wave runs the samples of those
+    // cells until every lane's next sample lies outside its own.  Lanes whose
+    // cells end early idle for the rest of that inner loop, which costs less
+    // than running the hand-over once for each group of lanes that leave
+    // together.  Membership is the exact march's: fractions p - b in [0, 1).
+    // Loop tests are wave votes and per-lane choices in the inner loop are
+    // 0/1 factors (an idle lane's fractions are taken as 0, so its terms are
+    // exact zeros): per-lane branches and loop exits cost scalar instructions.
+    // Each lane's samples, sums and hand-overs are those of a plain per-sample
+    // march.
     template <class Dep, class Go>
     __device__ __forceinline__ void run(const Geo1 &g, const VolDev &vol, float step, int axis, Dep &dep, Go go) {
-        if (!live) return;
         const int N = r.N;
         const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
         const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
-        // The sample position is computed one step ahead so that the slab test
-        // sits at the top of the loop (one loop exit, before any state
-        // changes, keeps the state in registers across the caller's slab loop).
-        float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy), pz = sample_pos(sz, alpha, riz);
-        // Fractions against the open cell: floor(p) is its base exactly when all
-        // three lie in [0, 1) (p - b is exact there; rounding is monotonic), i.e.
-        // their bits, unsigned, are below 0x3f800000 (a cell not yet loaded has
-        // a base of -1e30).  Only the slab axis needs the floor for the test.
-        float fx = px - c.bx, fy = py - c.by, fz = pz - c.bz;
-        bool same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
-        // one branch-free continuation test, evaluated at the bottom of the loop
-        bool cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
-        while (cont) {
-            // A lane whose next sample lies in another cell waits until at
-            // least half of the lanes still marching wait too (or all of them
-            // do), and then they change cells together.  The hand-over block
-            // costs the whole wave its full length whenever any lane runs it,
-            // and some lane leaves its cell on most samples, so running it for
-            // many lanes at once saves most of its executions.  Waiting only
-            // delays a lane's next sample; every lane's sequence of samples,
-            // sums and hand-overs is unchanged.
-            const uint64_t marching = __ballot(1);
-            const uint64_t waiting = __ballot(!same);
-            const bool change_now = 2 * __popcll(waiting) >= __popcll(marching);
-            if (!same && change_now) {
-                const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+        bool act = live;
+        while (__ballot(act) != 0) {
+            // the lane's current sample and its cell
+            const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
+                        pz = sample_pos(sz, alpha, riz);
+            const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
+            act = act & (t < N) & (alpha <= r.mxv) & go(axis == 0 ? bx : (axis == 1 ? by : bz));
+            if (act && !(bx == c.bx && by == c.by && bz == c.bz)) {
                 // one step to a face neighbour (in clamped cell coordinates): carry
                 // the shared face; anything else: hand over all 8 corners
                 int dd[3] = {0, 0, 0};
@@ -376,12 +370,14 @@ struct AdjRay {
                 }
                 c.load(vol, bx, by, bz);
                 set_h();
-                fx = px - bx;
-                fy = py - by;
-                fz = pz - bz;
-                same = true;
             }
-            if (same) {
+            // the samples of the cell; (ex, ey, ez) are the fractions of the
+            // lane's next sample against the open cell
+            bool in = act;
+            float ex = px - c.bx, ey = py - c.by, ez = pz - c.bz;
+            do {
+                const float gm = in ? 1.0f : 0.0f;
+                const float fx = in ? ex : 0.0f, fy = in ? ey : 0.0f, fz = in ? ez : 0.0f;
                 float w[8];
                 weights8(fx, fy, fz, w);
                 // t == 0 or t == N-1 (t < N), as one unsigned comparison
@@ -396,7 +392,7 @@ struct AdjRay {
                     sw = 0.0f;
 #pragma unroll
                     for (int d = 0; d < 8; d++) sw = fmaf(w[d], h[d], sw);
-                    if (POSE) {
+                    if (POSE && in) {
                         // label-gradient term of a cell whose labels differ (rare)
                         const float rho = trilerp(c.d0, c.d1, w);
                         float gh[3];
@@ -409,7 +405,7 @@ struct AdjRay {
                         }
                     }
                 }
-                const float G = wt * sw;
+                const float G = gm * (wt * sw);
 #pragma unroll
                 for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
                 if (POSE) {
@@ -417,19 +413,16 @@ struct AdjRay {
 #pragma unroll
                     for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
                 }
-                alpha += step;
-                ++t;
-                px = sample_pos(sx, alpha, rix);
-                py = sample_pos(sy, alpha, riy);
-                pz = sample_pos(sz, alpha, riz);
-                fx = px - c.bx;
-                fy = py - c.by;
-                fz = pz - c.bz;
-                same = max(max(__float_as_uint(fx), __float_as_uint(fy)), __float_as_uint(fz)) < 0x3f800000u;
-                cont = (t < N) & (alpha <= r.mxv) & (same | go(floorf(axis == 0 ? px : (axis == 1 ? py : pz))));
-            }
+                alpha = fmaf(gm, step, alpha);
+                t += in ? 1 : 0;
+                ex = sample_pos(sx, alpha, rix) - c.bx;
+                ey = sample_pos(sy, alpha, riy) - c.by;
+                ez = sample_pos(sz, alpha, riz) - c.bz;
+                in = in & (max(max(__float_as_uint(ex), __float_as_uint(ey)), __float_as_uint(ez)) < 0x3f800000u) &
+                     (t < N) & (alpha <= r.mxv);
+            } while (__ballot(in) != 0);
         }
-        if (!(t < N && alpha <= r.mxv)) {
+        if (live && !(t < N && alpha <= r.mxv)) {
             flush(vol, dep);
             live = false;
         }
@@ -583,11 +576,12 @@ __device__ __forceinline__ void ray_through(const float *W, float fu, float fv, 
 }
 
 // Occupancy targets that fit without (or, for more materials, with a few)
-// register spills: 5 waves per SIMD for the density gradient alone, 4 with
-// the pose sums.
+// register spills: 5 waves per SIMD for the density gradient alone, 3 with
+// the pose sums (density, pose moments and the carried face: 4 waves spill in
+// the cell loop and ran 4 % slower).
 template <int POSE>
 struct AdjOcc {
-    static constexpr int waves = POSE ? 4 : 5;
+    static constexpr int waves = POSE ? 3 : 5;
 };
 
 template <int M, bool ATTEN, bool POSE>
@@ -716,7 +710,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE
 // pose sums as k_adjoint_jobs<..., true>, without any density scatter, so no
 // slabs or LDS box.
 template <int M, bool ATTEN>
-__global__ __launch_bounds__(256) void k_adjoint_pose_jobs(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint_pose_jobs(const JobDev *__restrict__ jobs,
                                                            const int2 *__restrict__ items,
                                                            const int *__restrict__ seg, int *counters,
                                                            const float *__restrict__ sw_jobs, VolDev vol,
