@@ -182,21 +182,18 @@ struct AdjRay {
     float carry[8];
     float hu;       // dL/d(ad_raw) of the material of a cell whose labels agree
     // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t alpha_t dL/dp_t, in FP64
-    // (thousands of terms of both signs), kept in LDS (S[k * 256] for k < 6,
-    // this thread's column) rather than registers.  Within a cell, dL/dp_t =
+    // (thousands of terms of both signs), in registers.  Within a cell, dL/dp_t =
     // G_t grad(rho)(p_t) [+ wt rho_t grad(h)(p_t) if the labels differ, h_d =
     // dL/d(ad_raw) of corner d's material], and grad(rho) along x is
     // sum_yz (D[1yz] - D[0yz]) w_yz, so sum_t G_t grad_x(rho) =
     // sum_yz (D[1yz] - D[0yz]) (acc[0yz] + acc[1yz]) (w_0yz + w_1yz = w_yz):
     // the G term needs no per-sample gradient.  All of this is linear in the
-    // sums, so a cell may be flushed in parts.  Each thread owns its slots, so
-    // the LDS atomic adds below never contend: they are used because they
-    // need no read back (no wait on the LDS round trip).
-    double *S;
+    // sums, so a cell may be flushed in parts.  (They were per-thread LDS
+    // slots while the pose kernel ran at 4 waves per SIMD; at 3 the registers
+    // are there, and the FP64 adds leave the LDS to the box atomics.)
+    double S[6];
 
-    __device__ __forceinline__ void add_S(int k, double v) {
-        __hip_atomic_fetch_add(S + k * 256, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    __device__ __forceinline__ void add_S(int k, double v) { S[k] += v; }
 
     __device__ __forceinline__ void set_h() {
         if (c.uni) hu = get_m<M>(graw, (int)c.l1);
@@ -223,7 +220,7 @@ struct AdjRay {
         hu = 0.0f;
         if (POSE)
 #pragma unroll
-            for (int k = 0; k < 6; k++) S[k * 256] = 0.0;
+            for (int k = 0; k < 6; k++) S[k] = 0.0;
         march_setup_v1(g, udx, vdx, step, r);
         live = r.trace;
         alpha = r.minA;
@@ -436,7 +433,7 @@ struct AdjRay {
         // dp/dt = -A, dp/dw = A [x - c]_x  ->  dL/dt = -A^T S0,
         // dL/dw = (A^T S0) x (s_w - c) + (A^T S1) x r_w
         const float *A = g.A;
-        const double S0[3] = {S[0], S[256], S[512]}, S1[3] = {S[768], S[1024], S[1280]};
+        const double S0[3] = {S[0], S[1], S[2]}, S1[3] = {S[3], S[4], S[5]};
         double V0[3], V1[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -489,13 +486,11 @@ __device__ __forceinline__ void backward_spectral(const Geo1 &g, const VolDev &v
 template <int M, bool ATTEN, bool POSE>
 __device__ __forceinline__ void adjoint_ray(const Geo1 &g, const VolDev &vol, const SpectrumDev &sp, float step,
                                             int air, int udx, int vdx, float gi, float gp, const float sw[3],
-                                            const float center[3], float *gD, double *pose_slots,
-                                            double (&gpose)[6]) {
+                                            const float center[3], float *gD, double (&gpose)[6]) {
     if (gi == 0.0f && gp == 0.0f) return;
     float graw[M];
     backward_spectral<M, ATTEN>(g, vol, sp, step, air, udx, vdx, gi, gp, graw);
     AdjRay<M, POSE> ray;
-    ray.S = pose_slots;
     ray.begin(g, vol, udx, vdx, step, graw);
     GlobalDep dep{gD};
     ray.run(g, vol, step, 0, dep, AllCells{});
@@ -525,7 +520,6 @@ struct AdjArgs {
 template <int M, bool ATTEN, bool POSE>
 __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, SpectrumDev sp, float step, int air,
                                                     AdjArgs a, float swx, float swy, float swz, int tiles16_x) {
-    __shared__ double pose_slots[POSE ? 6 * 256 : 1];
     // tiles in row-major order, dispatched round-robin over the XCDs, so
     // every XCD gets a share of every band of rows (of ray lengths)
     const int lb = blockIdx.x;
@@ -538,7 +532,7 @@ __global__ __launch_bounds__(256) void k_adjoint_v1(Geo1 g, VolDev vol, Spectrum
         const int64_t idx = (int64_t)vdx * g.width + udx;
         const float sw[3] = {swx, swy, swz};
         adjoint_ray<M, ATTEN, POSE>(g, vol, sp, step, air, udx, vdx, a.gI ? a.gI[idx] : 0.0f,
-                                    a.gP ? a.gP[idx] : 0.0f, sw, a.c, a.gD, pose_slots + threadIdx.x, gpose);
+                                    a.gP ? a.gP[idx] : 0.0f, sw, a.c, a.gD, gpose);
     }
     if (POSE) flush_pose_wave(gpose, a.gPose);
 }
@@ -591,7 +585,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE
                                                       float step, int air, AdjArgs a, double *pose64) {
     __shared__ float box[ADJ_BOX_CELLS];
     __shared__ double s_pose[4][6];
-    __shared__ double pose_slots[POSE ? 6 * 256 : 1];
     for (int i = threadIdx.x; i < ADJ_BOX_CELLS; i += blockDim.x) box[i] = 0.0f;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -606,7 +599,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AdjOcc<POSE
             gp = a.gP ? a.gP[o] : 0.0f;
         }
         AdjRay<M, POSE> ray;
-        ray.S = pose_slots + threadIdx.x;
         ray.live = false;
         ray.r.trace = false;
         if (gi != 0.0f || gp != 0.0f) {
@@ -717,7 +709,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                            SpectrumDev sp, float step, int air, AdjArgs a,
                                                            double *pose64) {
     __shared__ double s_pose[4][6];
-    __shared__ double pose_slots[6 * 256];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for_each_block_item(jobs, items, seg, counters, [&](int j, const JobDev &J, int tile) __attribute__((always_inline)) {
         const int tx = tile % J.tiles_x, ty = tile / J.tiles_x;
@@ -743,7 +734,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 backward_spectral<M, ATTEN>(J.g, vol, sp, step, air, udx, vdx, gi, gp, graw);
             }
             AdjRay<M, true> ray;
-            ray.S = pose_slots + threadIdx.x;
             ray.begin(J.g, vol, udx, vdx, step, graw);
             NullDep dep;
             ray.run(J.g, vol, step, 0, dep, AllCells{});
