@@ -293,58 +293,148 @@ __device__ __forceinline__ void sample_v1(const VolDev &vol, Cell &c, float (&ad
 }
 
 // The fast march (DRR_MARCH_FAST): the samples t in [t, tlim), all of weight 1,
-// with the same FP32 alpha sequence and the same sequential FP32 additions into
-// the area densities as sample_v1, but with each sample's density formed far
-// more cheaply than by the exact filter.  Along the ray the trilinear density
-// of a cell is a cubic in the distance u = alpha - a0 from the sample where the
-// ray entered the cell; on entering a cell its four coefficients are formed
-// from the 8 corner densities, the entry fractions and the ray direction, with
-// the distance to the exit face, and each sample then costs one subtraction, a
-// comparison with the exit distance and three multiply-adds.  The entry sample
-// uses the exact march's position and floor; the samples after it are placed
-// by the exact alpha sequence (u is exact) on the exact line instead of at
-// their rounded FP32 positions, and a sample within rounding of a face may
-// take either cell (the density is continuous there).  A cell whose labels
-// differ takes the exact filter's formulas.  Each sample's value thus differs
-// from the exact march's by a few roundings, with both signs, and the sums
-// stay within about 1e-7 relative of the exact march's; nothing depends on
-// the launch or on other lanes, so results are the same from run to run and
+// with the same FP32 alpha sequence as sample_v1 and, in cells whose labels
+// agree, the same sequential FP32 additions into the area densities, but with
+// each sample's density formed far more cheaply than by the exact filter.
+// Along the ray the trilinear density of a cell is a cubic in the distance
+// u = alpha - a0 from the sample where the ray entered the cell; on entering a
+// cell its four coefficients are formed from the 8 corner densities, the entry
+// fractions and the ray direction, with the distance to the exit face, and each
+// sample then costs three multiply-adds and the loop's bookkeeping.  The entry
+// sample uses the exact march's position and floor; the samples after it are
+// placed by the exact alpha sequence (u is exact) on the exact line instead of
+// at their rounded FP32 positions, and a sample within rounding of a face may
+// take either cell (the density is continuous there).  A cell with two labels
+// splits the density by the second label's fraction, also a cubic (fast_run);
+// a cell with more takes the exact filter's formulas.  Each sample's value thus
+// differs from the exact march's by a few roundings, with both signs, and the
+// sums stay within about 1e-7 relative of the exact march's; nothing depends
+// on the launch or on other lanes, so results are the same from run to run and
 // from kernel to kernel.
+
+// Coefficients of the trilinear interpolant of corner values (v0 = corners
+// 0..3, v1 = 4..7, corner d = dx + 2 dy + 4 dz) along the ray p(u) = p0 + u r
+// from fractions f = p0 - b: value(u) = ((k3 u + k2) u + k1) u + k0.
+__device__ __forceinline__ void cell_cubic(const float4 &v0, const float4 &v1, float fx, float fy, float fz,
+                                           float rx, float ry, float rz, float &k0, float &k1, float &k2,
+                                           float &k3) {
+    // x: e_yz(u) = A_yz + B_yz u, the lerp of corners (0yz, 1yz)
+    const float X00 = v0.y - v0.x, X10 = v0.w - v0.z, X01 = v1.y - v1.x, X11 = v1.w - v1.z;
+    const float A00 = __builtin_fmaf(fx, X00, v0.x), A10 = __builtin_fmaf(fx, X10, v0.z);
+    const float A01 = __builtin_fmaf(fx, X01, v1.x), A11 = __builtin_fmaf(fx, X11, v1.z);
+    const float B00 = rx * X00, B10 = rx * X10, B01 = rx * X01, B11 = rx * X11;
+    // y: g_z(u) = G_z0 + G_z1 u + G_z2 u^2
+    const float P0 = A10 - A00, Q0 = B10 - B00, P1 = A11 - A01, Q1 = B11 - B01;
+    const float G00 = __builtin_fmaf(fy, P0, A00), G01 = __builtin_fmaf(fy, Q0, __builtin_fmaf(ry, P0, B00));
+    const float G02 = ry * Q0;
+    const float G10 = __builtin_fmaf(fy, P1, A01), G11 = __builtin_fmaf(fy, Q1, __builtin_fmaf(ry, P1, B01));
+    const float G12 = ry * Q1;
+    // z: the cubic
+    const float D0 = G10 - G00, D1 = G11 - G01, D2 = G12 - G02;
+    k0 = __builtin_fmaf(fz, D0, G00);
+    k1 = __builtin_fmaf(fz, D1, __builtin_fmaf(rz, D0, G01));
+    k2 = __builtin_fmaf(fz, D2, __builtin_fmaf(rz, D1, G02));
+    k3 = rz * D2;
+}
+
 struct FastCell {
-    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;  // density = ((c3 u + c2) u + c1) u + c0
+    // density = ((c3 u + c2) u + c1) u + c0 (fast_run keeps the second label's
+    // corner indicators here instead in a two-label cell, zeros in a cell with more)
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
     float a0 = 0.f;    // alpha of the entry sample (u = alpha - a0)
-    float uex = -1.f;  // u at the exit face
+    float nex = 0.f;   // distance to the exit face, in steps
 
     // Enter cell c at alpha with fractions (fx, fy, fz) (those of the exact
-    // march), the ray direction r in voxels per mm and its reciprocal ir.
+    // march) and the ray direction r in voxels per mm.  Per axis, the exit
+    // face is reached after fma(-f, j, h) steps with j = 1 / (r step) and
+    // h = j going up, 0 going down (j = 0, h = +inf where r = 0).
     __device__ __forceinline__ void enter(const Cell &c, float alpha, float fx, float fy, float fz, float rx,
-                                          float ry, float rz, float irx, float iry, float irz) {
+                                          float ry, float rz, float jx, float jy, float jz, float hx, float hy,
+                                          float hz) {
         a0 = alpha;
-        // x: e_yz(u) = A_yz + B_yz u, the lerp of corners (0yz, 1yz)
-        const float X00 = c.d0.y - c.d0.x, X10 = c.d0.w - c.d0.z, X01 = c.d1.y - c.d1.x, X11 = c.d1.w - c.d1.z;
-        const float A00 = __builtin_fmaf(fx, X00, c.d0.x), A10 = __builtin_fmaf(fx, X10, c.d0.z);
-        const float A01 = __builtin_fmaf(fx, X01, c.d1.x), A11 = __builtin_fmaf(fx, X11, c.d1.z);
-        const float B00 = rx * X00, B10 = rx * X10, B01 = rx * X01, B11 = rx * X11;
-        // y: g_z(u) = G_z0 + G_z1 u + G_z2 u^2
-        const float P0 = A10 - A00, Q0 = B10 - B00, P1 = A11 - A01, Q1 = B11 - B01;
-        const float G00 = __builtin_fmaf(fy, P0, A00), G01 = __builtin_fmaf(fy, Q0, __builtin_fmaf(ry, P0, B00));
-        const float G02 = ry * Q0;
-        const float G10 = __builtin_fmaf(fy, P1, A01), G11 = __builtin_fmaf(fy, Q1, __builtin_fmaf(ry, P1, B01));
-        const float G12 = ry * Q1;
-        // z: the cubic
-        const float D0 = G10 - G00, D1 = G11 - G01, D2 = G12 - G02;
-        c0 = __builtin_fmaf(fz, D0, G00);
-        c1 = __builtin_fmaf(fz, D1, __builtin_fmaf(rz, D0, G01));
-        c2 = __builtin_fmaf(fz, D2, __builtin_fmaf(rz, D1, G02));
-        c3 = rz * D2;
-        // exit: the first face a fraction reaches (1 going up, 0 going down;
-        // an axis the ray does not move along gives +inf)
-        const float ux = ((rx < 0.0f ? 0.0f : 1.0f) - fx) * irx;
-        const float uy = ((ry < 0.0f ? 0.0f : 1.0f) - fy) * iry;
-        const float uz = ((rz < 0.0f ? 0.0f : 1.0f) - fz) * irz;
-        uex = fminf(ux, fminf(uy, uz));
+        cell_cubic(c.d0, c.d1, fx, fy, fz, rx, ry, rz, c0, c1, c2, c3);
+        nex = fminf(__builtin_fmaf(-fx, jx, hx), fminf(__builtin_fmaf(-fy, jy, hy), __builtin_fmaf(-fz, jz, hz)));
     }
 };
+
+// Bit 3 of nibble d is set where nibble d of x is nonzero (exact per nibble:
+// the low three bits plus 7 carry into bit 3 and never into the next nibble).
+__device__ __forceinline__ uint32_t nz_nibbles(uint32_t x) {
+    return (((x & 0x77777777u) + 0x77777777u) | x) & 0x88888888u;
+}
+
+// The samples of the entered cells (fast_run): n for this lane, the density
+// cubic in fc (tw: a two-label cell, whose corner data fc and e hold instead)
+// and the two-label formulas where tw (TWO: some lane of the wave has a
+// two-label cell; the two forms are separate loops).
+template <bool TWO>
+__device__ __forceinline__ void fast_cell(const Cell &c, const FastCell &fc, Accum &a, float &sq, float e0,
+                                          float e1, float e2, float e3, bool tw, int n, float sx, float sy,
+                                          float sz, float rix, float riy, float riz, float step, float &alpha) {
+    float u = 0.0f;
+    for (int k = 0;; ++k) {
+        const bool in = k < n;
+        if (__ballot(in) == 0) break;
+        float rho = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(fc.c3, u, fc.c2), u, fc.c1), u, fc.c0);
+        const float g = in ? 1.0f : 0.0f;
+        if (TWO) {
+            // two labels: density and q-fraction at the sample's FP32 position
+            // (that of the exact filter), each by nested lerps
+            const float fx = __builtin_fmaf(alpha, rix, sx) - (c.bx + 1.0f);
+            const float fy = __builtin_fmaf(alpha, riy, sy) - (c.by + 1.0f);
+            const float fz = __builtin_fmaf(alpha, riz, sz) - (c.bz + 1.0f);
+            const float a00 = __builtin_fmaf(fx, c.d0.y - c.d0.x, c.d0.x);
+            const float a10 = __builtin_fmaf(fx, c.d0.w - c.d0.z, c.d0.z);
+            const float a01 = __builtin_fmaf(fx, c.d1.y - c.d1.x, c.d1.x);
+            const float a11 = __builtin_fmaf(fx, c.d1.w - c.d1.z, c.d1.z);
+            const float b0 = __builtin_fmaf(fy, a10 - a00, a00), b1 = __builtin_fmaf(fy, a11 - a01, a01);
+            const float x = __builtin_fmaf(fz, b1 - b0, b0);
+            const float i00 = __builtin_fmaf(fx, e0, fc.c0), i10 = __builtin_fmaf(fx, e1, fc.c1);
+            const float i01 = __builtin_fmaf(fx, e2, fc.c2), i11 = __builtin_fmaf(fx, e3, fc.c3);
+            const float j0 = __builtin_fmaf(fy, i10 - i00, i00), j1 = __builtin_fmaf(fy, i11 - i01, i01);
+            const float fq = __builtin_fmaf(fz, j1 - j0, j0);
+            const float xq = x * fq;
+            sq = __builtin_fmaf(g, tw ? xq : 0.0f, sq);
+            rho = tw ? x - xq : rho;
+        }
+        a.cur = __builtin_fmaf(g, rho, a.cur);
+        alpha = __builtin_fmaf(g, step, alpha);
+        u = alpha - fc.a0;
+    }
+}
+
+// The samples of a cell with three or more labels (ex), after fast_cell has
+// taken the cell's samples for the other lanes: the exact filter's formulas at
+// the same FP32 alpha sequence, re-walked from the entry sample a0, added into
+// ad[] sample by sample (the lane has no running sum in such a cell).
+template <int M>
+__device__ __forceinline__ void fast_exact(const Cell &c, float (&ad)[M], bool ex, int n, float a0, float sx,
+                                           float sy, float sz, float rix, float riy, float riz, float step) {
+    float al = a0;
+    for (int k = 0;; ++k) {
+        const bool in = (k < n) & ex;
+        if (__ballot(in) == 0) break;
+        if (in) {
+            const float fx = __builtin_fmaf(al, rix, sx) - (c.bx + 1.0f);
+            const float fy = __builtin_fmaf(al, riy, sy) - (c.by + 1.0f);
+            const float fz = __builtin_fmaf(al, riz, sz) - (c.bz + 1.0f);
+            float w[8];
+            weights8(fx, fy, fz, w);
+            const float x = trilerp(c.d0, c.d1, w);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const uint32_t xm = c.l0 ^ (0x11111111u * (uint32_t)m);
+                if (((xm - 0x11111111u) & ~xm & 0x88888888u) != 0u) {
+                    float sm = 0.0f;
+#pragma unroll
+                    for (int d = 0; d < 8; d++) sm += (corner_label(c.l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
+                    ad[m] = __builtin_fmaf(x, sm, ad[m]);
+                }
+            }
+        }
+        al += step;
+    }
+}
 
 template <int M>
 __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &fc, float (&ad)[M], Accum &a,
@@ -365,74 +455,120 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     // a loop this short.  With g = 1, fma(g, rho, cur) = cur + rho and
     // fma(g, step, alpha) = alpha + step exactly; with g = 0 both leave the
     // value unchanged.
+    //
+    // Materials: a cell whose labels agree adds each sample's density to the
+    // running sum of its material (Accum).  A cell with exactly two labels
+    // (nearly every cell on a material boundary) forms each sample's density
+    // and the fraction f_q of its second label q at the sample's FP32 position,
+    // as the exact filter does (across a boundary the density changes by up to
+    // several g/cm^3 per voxel, so that position's rounding shows at 1e-5 in
+    // the sums, unlike in a cell of one material), each by nested lerps:
+    // rho (1 - f_q) goes to the running sum of the first material and rho f_q
+    // to a second running sum sq of ad[q], written back when q or the first
+    // material changes and when the run ends; so every term is still added into
+    // its material's area density one sample at a time, as the exact filter
+    // adds it.  A cell with three or more labels takes the exact filter's
+    // formulas, its density cubic set to zero.  In a wave with no two-label
+    // cell the two-label formulas are skipped; a lane in a one-label cell
+    // selects its cubic's value and adds zero to sq, so its results do not
+    // depend on the other lanes.
     if (__ballot(t < tlim) == 0) return;
-    // reciprocals for the exit distances (1 ulp is plenty there; +-inf for an
-    // axis the ray does not move along), kept in registers (an empty asm stops
-    // the compiler from forming them again on every cell entry)
-    float irx = __builtin_amdgcn_rcpf(rix), iry = __builtin_amdgcn_rcpf(riy), irz = __builtin_amdgcn_rcpf(riz);
-    asm volatile("" : "+v"(irx), "+v"(iry), "+v"(irz));
+    // exit distances in steps (reciprocals to 1 ulp are plenty there), kept in
+    // registers (an empty asm stops the compiler from forming them again on
+    // every cell entry)
     const float istep = 1.0f / step;
+    float jx = rix == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(rix) * istep;
+    float jy = riy == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(riy) * istep;
+    float jz = riz == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(riz) * istep;
+    float hx = rix == 0.0f ? INFINITY : (rix < 0.0f ? 0.0f : jx);
+    float hy = riy == 0.0f ? INFINITY : (riy < 0.0f ? 0.0f : jy);
+    float hz = riz == 0.0f ? INFINITY : (riz < 0.0f ? 0.0f : jz);
+    asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(hx), "+v"(hy), "+v"(hz));
+    float sq = 0.0f;  // ad[q] while q (the second material of two-label cells) is >= 0
+    int q = -1;
+    // two labels: the q-indicators' x-differences (read only where tw, so not
+    // cleared for other cells)
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
     do {
         int n = 0;        // samples of the entered cell
-        float uf = 0.0f;  // 1 in a cell whose labels agree
+        bool ex = false;  // three or more labels: the exact filter
+        bool tw = false;  // two labels
         if (t < tlim) {
             const float px = sample_pos(sx, alpha, rix), py = sample_pos(sy, alpha, riy),
                         pz = sample_pos(sz, alpha, riz);
             const float bx = floorf(px), by = floorf(py), bz = floorf(pz);
             c.load(vol, bx, by, bz);
-            fc.enter(c, alpha, px - bx, py - by, pz - bz, rix, riy, riz, irx, iry, irz);
-            const int ncm = (int)c.l1;  // the common material, -1 if the labels differ
+            const float fx = px - bx, fy = py - by, fz = pz - bz;
+            fc.enter(c, alpha, fx, fy, fz, rix, riy, riz, jx, jy, jz, hx, hy, hz);
+            int ncm = (int)c.l1;  // the first material (the common one in a one-label cell)
+            int qn = -1;          // the second material of a two-label cell
+            if (!c.uni) {
+                const uint32_t l0 = c.l0;
+                const uint32_t la = l0 & 0xfu;
+                const uint32_t xa = l0 ^ (la * 0x11111111u);  // nonzero nibbles: corners not la
+                const uint32_t lb = (l0 >> (__builtin_ctz(xa) & ~3u)) & 0xfu;
+                const uint32_t xb = l0 ^ (lb * 0x11111111u);
+                const uint32_t na = nz_nibbles(xa), nb = nz_nibbles(xb);
+                if ((na & nb) == 0u) {
+                    // the first material keeps the running sum when it already has it
+                    const bool sw = (int)lb == a.cm;
+                    ncm = (int)(sw ? lb : la);
+                    qn = (int)(sw ? la : lb);
+                    const uint32_t iq = sw ? na : nb;  // bit set: corner not q
+                    // q-indicators of the corners: bases of the x-lerps in fc.c0..c3,
+                    // their differences in e0..e3
+                    fc.c0 = (iq & 0x8u) ? 0.f : 1.f;
+                    fc.c1 = (iq & 0x800u) ? 0.f : 1.f;
+                    fc.c2 = (iq & 0x80000u) ? 0.f : 1.f;
+                    fc.c3 = (iq & 0x8000000u) ? 0.f : 1.f;
+                    e0 = ((iq & 0x80u) ? 0.f : 1.f) - fc.c0;
+                    e1 = ((iq & 0x8000u) ? 0.f : 1.f) - fc.c1;
+                    e2 = ((iq & 0x800000u) ? 0.f : 1.f) - fc.c2;
+                    e3 = ((iq & 0x80000000u) ? 0.f : 1.f) - fc.c3;
+                    tw = true;
+                } else {
+                    ex = true;
+                    fc.c0 = fc.c1 = fc.c2 = fc.c3 = 0.0f;
+                }
+            }
+            if (q >= 0 && (ncm != a.cm || (qn >= 0 && qn != q))) {
+                set_m<M>(ad, q, sq);
+                q = -1;
+            }
             if (ncm != a.cm) {
                 if (a.cm >= 0) set_m<M>(ad, a.cm, a.cur);
                 a.cm = ncm;
                 a.cur = get_m<M>(ad, ncm);
             }
-            uf = c.uni ? 1.0f : 0.0f;
-            // ceil(uex / step), clamped to [1, tlim - t] after the conversion
-            // (which saturates infinities and takes NaN to 0), so every entry
-            // takes at least one sample
-            n = min(max((int)ceilf(fc.uex * istep), 1), tlim - t);
-        }
-        float u = 0.0f;
-        for (int k = 0;; ++k) {
-            const bool in = k < n;
-            if (__ballot(in) == 0) break;
-            if (in & (uf == 0.0f)) {
-                // (alpha and the labels through an empty asm: the fractions and
-                // the label tests stay in this rare branch instead of being
-                // hoisted into every sample or every cell entry)
-                float al = alpha;
-                uint32_t l0 = c.l0;
-                asm volatile("" : "+v"(al), "+v"(l0));
-                const float fx = __builtin_fmaf(al, rix, sx) - (c.bx + 1.0f);
-                const float fy = __builtin_fmaf(al, riy, sy) - (c.by + 1.0f);
-                const float fz = __builtin_fmaf(al, riz, sz) - (c.bz + 1.0f);
-                float w[8];
-                weights8(fx, fy, fz, w);
-                const float x = trilerp(c.d0, c.d1, w);
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    const uint32_t xm = l0 ^ (0x11111111u * (uint32_t)m);
-                    if (((xm - 0x11111111u) & ~xm & 0x88888888u) != 0u) {
-                        float sm = 0.0f;
-#pragma unroll
-                        for (int d = 0; d < 8; d++) sm += (corner_label(l0, c.l1, d) == (uint32_t)m) ? w[d] : 0.0f;
-                        ad[m] = __builtin_fmaf(x, sm, ad[m]);
-                    }
-                }
+            if (qn >= 0 && q < 0) {
+                q = qn;
+                sq = get_m<M>(ad, qn);
             }
-            const float rho = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(fc.c3, u, fc.c2), u, fc.c1), u, fc.c0);
-            const float g = in ? 1.0f : 0.0f;
-            a.cur = __builtin_fmaf(g * uf, rho, a.cur);
-            alpha = __builtin_fmaf(g, step, alpha);
-            u = alpha - fc.a0;
+            // the samples before the exit face, clamped to [1, tlim - t] after
+            // the conversion (which saturates infinities and takes NaN to 0), so
+            // every entry takes at least one sample
+            n = min(max((int)ceilf(fc.nex), 1), tlim - t);
         }
+        if (__ballot(tw) != 0)
+            fast_cell<true>(c, fc, a, sq, e0, e1, e2, e3, tw, n, sx, sy, sz, rix, riy, riz, step, alpha);
+        else
+            fast_cell<false>(c, fc, a, sq, e0, e1, e2, e3, tw, n, sx, sy, sz, rix, riy, riz, step, alpha);
+        if (__ballot(ex) != 0) fast_exact<M>(c, ad, ex, n, fc.a0, sx, sy, sz, rix, riy, riz, step);
         t += n;
     } while (__ballot(t < tlim) != 0);
+    if (q >= 0) set_m<M>(ad, q, sq);
+    // sample_v1 adds a mixed cell's terms into ad[] directly and expects no
+    // running sum there: write a two-label cell's first material back
+    if (!c.uni && a.cm >= 0) {
+        set_m<M>(ad, a.cm, a.cur);
+        a.cm = -1;
+        a.cur = 0.0f;
+    }
 }
 
 // Occupancy target of the march kernels: up to 5 materials the march fits in
-// 64 VGPRs (8 waves per SIMD) without spills; more materials get more registers.
+// 64 VGPRs (8 waves per SIMD) without spills in its sample loops; more
+// materials get more registers.
 template <int M>
 struct MarchOcc {
     static constexpr int waves = M <= 5 ? 8 : 5;
