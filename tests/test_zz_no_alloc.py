@@ -3,7 +3,8 @@ against drr_project_jobs (job tables in fresh device allocations).
 
 Bar: bit-identical intensity, photon_prob and area densities on two calls in a
 row, the second with a smaller batch (the scratch keeps the first call's
-tables), and a batch larger than the scratch is refused on the host.
+tables), with the exact and the fast march, and a batch larger than the scratch
+is refused on the host.
 """
 import ctypes as C
 
@@ -12,14 +13,14 @@ import pytest
 
 from deepdrr_amd import _abi, phantoms
 from deepdrr_amd import projector as P
-from deepdrr_amd._abi_types import Job, Scene
+from deepdrr_amd._abi_types import MARCH_EXACT, MARCH_FAST, Job, Scene
 from deepdrr_amd.material import attenuation_table
 from deepdrr_amd.spectral_data import spectrum_tables, synthetic_120kv
 
 pytestmark = pytest.mark.gpu
 
 
-def _scene():
+def _scene(march):
     vols = [phantoms.small_ct(24, 0), phantoms.small_ct(16, 1), phantoms.small_ct(20, 2)]
     vols[1].place_center([5.0, 3.0, -4.0])
     mats = sorted({m for v in vols for m in v.materials[0]})
@@ -27,7 +28,7 @@ def _scene():
     mu = attenuation_table(mats, E)
     ctx = C.c_void_p()
     _abi.call("drr_ctx_create", 0, C.byref(ctx))
-    sc = Scene(step=0.1, n_materials=len(mats), attenuate_outside=1, air_index=mats.index("air"))
+    sc = Scene(step=0.1, n_materials=len(mats), attenuate_outside=1, air_index=mats.index("air"), march=march)
     _abi.call("drr_ctx_set_scene", ctx, C.byref(sc))
     _abi.call("drr_ctx_set_spectrum", ctx, len(E), E.ctypes.data, pdf.ctypes.data, mu.ctypes.data)
     handles = []
@@ -74,14 +75,15 @@ def _run(name, ctx, varr, n_vols, jobs, n, total, M):
     return [t.cpu().numpy().view(np.uint32) for t in (I, Pp, AD)]
 
 
-def test_noalloc_matches_allocating_entry_point(gpu):
-    ctx, handles, vols, M = _scene()
+@pytest.mark.parametrize("march", [MARCH_EXACT, MARCH_FAST], ids=["exact", "fast"])
+def test_noalloc_matches_allocating_entry_point(gpu, march):
+    ctx, handles, vols, M = _scene(march)
     varr = (C.c_void_p * 3)(*[h.value for h in handles])
     try:
         # a larger batch, then a smaller one over the scratch the first left behind
         for n, seed, sizes, multi in [(12, 3, [(40, 33), (24, 20), (17, 9)], True), (4, 8, [(9, 7), (16, 13)], False)]:
             jobs, total = _jobs(vols, n, seed, sizes, multi)
-            print(f"noalloc: {n} jobs, {total} pixels", flush=True)
+            print(f"noalloc: march {march}, {n} jobs, {total} pixels", flush=True)
             ref = _run("drr_project_jobs", ctx, varr, 3, jobs, n, total, M)
             got = _run("drr_project_jobs_noalloc", ctx, varr, 3, jobs, n, total, M)
             for a, b, what in zip(ref, got, ("intensity", "photon_prob", "area_density")):
