@@ -321,6 +321,50 @@ struct AdjRay {
         }
     }
 
+    // Sums of a one-label cell's n samples of weight 1 in closed form.  The
+    // samples sit at u_k = k h (k < n) from the entry sample, whose fractions
+    // against the cell are f.  About the run's midpoint u_m = h (n - 1) / 2,
+    // with v = u - u_m, corner d = (a, b, c)'s weight is X_a(v) Y_b(v) Z_c(v),
+    // X_1 = x_1 + r_x v with x_1 = f_x + r_x u_m and X_0 = 1 - X_1 (so
+    // x_0' = -r_x, x_1' = r_x), and likewise along y and z.  The odd central
+    // sums of v vanish, so with C2 = sum v^2 = h^2 n (n^2 - 1) / 12 and
+    // C4 = sum v^4 = h^4 n (n^2 - 1) (3 n^2 - 7) / 240:
+    //   sum_k w_d = n xyz + C2 (x'y'z + x'yz' + xy'z'),
+    //   sum_k alpha_k w_d = alpha_m sum_k w_d + C2 (x'yz + xy'z + xyz') + C4 x'y'z',
+    // with alpha_m the midpoint's alpha.  Both are added times hu (the
+    // material's dL/d(ad_raw)), as the per-sample loop adds G w_d with G = hu
+    // times the weights' sum, which is 1 to rounding.
+    __device__ __forceinline__ void cell_sums(int n, float h, float am, float fx, float fy, float fz, float rx,
+                                              float ry, float rz) {
+        const float nf = (float)n;
+        const float um = 0.5f * h * (nf - 1.0f);
+        const float x1 = fmaf(rx, um, fx), y1 = fmaf(ry, um, fy), z1 = fmaf(rz, um, fz);
+        const float X[2] = {1.0f - x1, x1}, Y[2] = {1.0f - y1, y1}, Z[2] = {1.0f - z1, z1};
+        const float Xd[2] = {-rx, rx}, Yd[2] = {-ry, ry}, Zd[2] = {-rz, rz};
+        const float h2 = h * h, q = fmaf(nf, nf, -1.0f);
+        const float C2 = h2 * nf * q * (1.0f / 12.0f);
+        const float C4 = h2 * h2 * nf * q * fmaf(3.0f * nf, nf, -7.0f) * (1.0f / 240.0f);
+        const float amid = am + um;
+        // per (b, c) pair: yz, y'z + yz', y'z'; then each corner (a, b, c)
+#pragma unroll
+        for (int bc = 0; bc < 4; bc++) {
+            const int b = bc & 1, c3 = bc >> 1;
+            const float yz = Y[b] * Z[c3], s1 = fmaf(Yd[b], Z[c3], Y[b] * Zd[c3]), qq = Yd[b] * Zd[c3];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int d = a + 2 * bc;
+                const float t2 = fmaf(Xd[a], s1, X[a] * qq);  // x'(y'z + yz') + x y'z'
+                const float W = fmaf(C2, t2, nf * (X[a] * yz));
+                acc[d] = fmaf(hu, W, acc[d]);
+                if (POSE) {
+                    const float t3 = fmaf(X[a], s1, Xd[a] * yz);  // x'yz + x(y'z + yz')
+                    const float A = fmaf(amid, W, fmaf(C2, t3, C4 * (Xd[a] * qq)));
+                    acc1[d] = fmaf(hu, A, acc1[d]);
+                }
+            }
+        }
+    }
+
     // March on while the floor of the sample position along `axis` of each
     // newly entered cell satisfies go(floor).  The open cell's sums are handed
     // over when the ray leaves it or ends.
@@ -342,6 +386,15 @@ struct AdjRay {
         const int N = r.N;
         const float sx = g.s[0], sy = g.s[1], sz = g.s[2];
         const float rix = r.ri[0], riy = r.ri[1], riz = r.ri[2];
+        // exit distances in steps: fma(-f, j, h) per axis, j = 1 / (r step),
+        // h = j going up, 0 going down (j = 0, h = +inf where r = 0)
+        const float istep = 1.0f / step;
+        const float jx = rix == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(rix) * istep;
+        const float jy = riy == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(riy) * istep;
+        const float jz = riz == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(riz) * istep;
+        const float hx = rix == 0.0f ? INFINITY : (rix < 0.0f ? 0.0f : jx);
+        const float hy = riy == 0.0f ? INFINITY : (riy < 0.0f ? 0.0f : jy);
+        const float hz = riz == 0.0f ? INFINITY : (riz < 0.0f ? 0.0f : jz);
         bool act = live;
         while (__ballot(act) != 0) {
             // the lane's current sample and its cell
@@ -368,10 +421,33 @@ struct AdjRay {
                 c.load(vol, bx, by, bz);
                 set_h();
             }
-            // the samples of the cell; (ex, ey, ez) are the fractions of the
-            // lane's next sample against the open cell
-            bool in = act;
+            // (ex, ey, ez): the fractions of the lane's next sample against the
+            // open cell
             float ex = px - c.bx, ey = py - c.by, ez = pz - c.bz;
+            // A one-label cell whose samples all have weight 1 and over which
+            // the FP32 alpha sequence is linear (its first two increments agree,
+            // so no tie alternates, and it stays in one binade) takes its samples
+            // at once (cell_sums).  The sample count comes from the exit distance,
+            // so a sample within rounding of a face may fall to either cell (the
+            // weights on the shared face agree); the last sample is kept for
+            // the loop below when the ray ends in the cell.
+            bool cf = false;
+            if (act && c.uni && t >= 1) {
+                const float nex = fminf(fmaf(-ex, jx, hx), fminf(fmaf(-ey, jy, hy), fmaf(-ez, jz, hz)));
+                const int n = min(max((int)ceilf(nex), 1), N - 1 - t);
+                const float a1 = alpha + step, s1 = a1 - alpha, s2 = (a1 + step) - a1;
+                const float an = fmaf((float)n, s1, alpha);  // alpha after the n samples
+                if (n >= 1 && s1 == s2 && (__float_as_uint(an) ^ __float_as_uint(alpha)) < 0x00800000u &&
+                    an - s1 <= r.mxv) {
+                    cell_sums(n, s1, alpha, ex, ey, ez, rix, riy, riz);
+                    alpha = an;
+                    t += n;
+                    cf = true;
+                }
+            }
+            // the other lanes' samples of the cell, one by one
+            bool in = act & !cf;
+            if (__ballot(in) == 0) continue;
             do {
                 const float gm = in ? 1.0f : 0.0f;
                 const float fx = in ? ex : 0.0f, fy = in ? ey : 0.0f, fz = in ? ez : 0.0f;
