@@ -80,6 +80,9 @@ class BatchPlan:
                   1 if neglog else 0, C.byref(self._batch))
         self.n_views = n
         self.total = total
+        self._copy_stream = None  # fetch_async: copy stream, staging buffer, last copy
+        self._staging = None
+        self._copy_done = None
 
     def launch(self, stream=None) -> None:
         s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
@@ -103,6 +106,40 @@ class BatchPlan:
         host = host.numpy()
         return [host[o:o + h * w].reshape(h, w) for o, (h, w) in zip(self.offsets, self.shapes)]
 
+    def fetch_async(self, which: str = "images") -> "PendingFetch":
+        """Start copying this launch's images to host memory without waiting.
+
+        The images go device-to-device into a staging buffer on the current
+        stream (a copy at HBM speed), then to page-locked host memory on a copy
+        stream of this plan's own, so the next ``launch`` runs while they cross
+        the link.  ``PendingFetch.result()`` waits for that copy and returns the
+        arrays as ``fetch`` does.  A later call first waits (on the device)
+        until the previous host copy has left the staging buffer."""
+        torch = self.torch
+        buf = {"images": self.images, "intensity": self.intensity, "photon_prob": self.photon_prob}[which]
+        if buf is None:
+            raise ValueError(f"{which} was not requested when the batch was planned")
+        if self._copy_stream is None:
+            self._copy_stream = torch.cuda.Stream(self.device)  # default priority
+        cur = torch.cuda.current_stream(self.device)
+        if self._staging is None or self._staging.numel() < buf.numel():
+            self._staging = torch.empty(buf.numel(), dtype=buf.dtype, device=self.device)
+        if self._copy_done is not None:
+            cur.wait_event(self._copy_done)
+        staging = self._staging[:buf.numel()]
+        staging.copy_(buf)
+        ready = cur.record_event()
+        host = torch.empty(buf.numel(), dtype=buf.dtype, pin_memory=True)
+        with torch.cuda.stream(self._copy_stream):
+            self._copy_stream.wait_event(ready)
+            host.copy_(staging, non_blocking=True)
+            done = self._copy_stream.record_event()
+        # the copy stream reads the staging buffer: keep torch's allocator from
+        # handing it out again before that copy is done
+        self._staging.record_stream(self._copy_stream)
+        self._copy_done = done
+        return PendingFetch(host, done, self.offsets, self.shapes)
+
     def __call__(self, which: Optional[str] = None):
         self.launch()
         out = self.fetch(which or ("images" if self.neglog else "intensity"))
@@ -120,6 +157,18 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class PendingFetch:
+    """A host copy started by ``BatchPlan.fetch_async``."""
+
+    def __init__(self, host, done, offsets, shapes):
+        self._host, self._done, self._offsets, self._shapes = host, done, offsets, shapes
+
+    def result(self) -> List[np.ndarray]:
+        self._done.synchronize()
+        host = self._host.numpy()
+        return [host[o:o + h * w].reshape(h, w) for o, (h, w) in zip(self._offsets, self._shapes)]
 
 
 def plan_for_projector(projector, projections: Sequence[geo.CameraProjection],

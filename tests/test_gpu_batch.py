@@ -111,6 +111,31 @@ def test_graph_replays_are_identical(gpu):
         _teardown(ctx, handles)
 
 
+def test_fetch_async_keeps_each_launch(gpu):
+    """fetch_async: each pending copy holds the images of the launch it follows,
+    bit for bit those fetch returns, even when the device buffer is overwritten
+    right after (as the next launch does) and several copies are in flight."""
+    ctx, handles, arrays, views, vidx, *_ = _setup()
+    try:
+        plan = BatchPlan(ctx, handles, views, vidx, neglog=True, tuning=(8, 8, 4, 8))
+        plan.launch()
+        ref = plan.fetch("images")
+        pend = []
+        for i in range(3):
+            plan.launch()
+            pend.append(plan.fetch_async("images"))
+            plan.images.fill_(float(i + 7))
+            pend.append(plan.fetch_async("images"))
+        for i in range(3):
+            for x, y in zip(pend[2 * i].result(), ref):
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"launch {i}"
+            for x in pend[2 * i + 1].result():
+                assert np.all(x == float(i + 7)), f"overwrite {i}"
+        plan.close()
+    finally:
+        _teardown(ctx, handles)
+
+
 def test_device_neglog_matches_host(gpu):
     import torch
 
