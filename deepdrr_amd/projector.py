@@ -119,7 +119,8 @@ class Projector:
         cross sections of ``mc.SYNTHETIC_MU60``), added to the intensity
         before noise and neglog; 0 (default) renders no scatter; and
         ``march``: ``"fast"`` (default) forms each sample's density from
-        per-cell cubics along the ray (images within 1e-5 relative of the
+        per-cell cubics along the ray (nested lerps in a cell with two labels,
+        the reference's filter with more; images within 1e-5 relative of the
         reference's filter, area densities within rounding of it) or
         ``"exact"`` (the reference's per-sample filter and FP32 operation
         order: area densities bit-identical to the oracle's transcription);

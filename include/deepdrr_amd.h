@@ -64,10 +64,12 @@ typedef struct {
 /* How a single-volume view is marched (drr_scene_t.march).  EXACT: the
  * reference's per-sample trilinear filter with its FP32 operation order
  * (area densities bit-identical to the oracle's transcription).  FAST: the same
- * samples and the same sequential FP32 sums, with each sample's density formed
- * by nested lerps from per-cell differences (values within rounding of EXACT,
- * images within 1e-5 relative).  Views over several volumes or with meshes are
- * marched exactly in both modes.  No counterpart in the reference. */
+ * samples, each term still added into its material's area density one sample
+ * at a time, with each sample's density formed from a per-cell cubic along the
+ * ray (a cell with two labels: nested lerps at the sample's position; with
+ * more: the exact filter), so values are within rounding of EXACT and images
+ * within 1e-5 relative.  Views over several volumes or with meshes are marched
+ * exactly in both modes.  No counterpart in the reference. */
 #define DRR_MARCH_EXACT 0
 #define DRR_MARCH_FAST 1
 
