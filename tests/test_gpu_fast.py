@@ -1,11 +1,13 @@
 """The fast march (DRR_MARCH_FAST) against the oracle's transcription.
 
-Bar (the fast march forms each sample's density by nested lerps instead of the
-exact 8-weight filter, so values differ from the transcription by rounding):
-intensity and photon_prob within 1e-5 relative of the transcription; every
-area density within 1e-5 of the pixel's total area density (the largest
-difference seen is printed); bit-identical from run to run; batched images
-bit-identical to one view per launch (the same march in every kernel).
+Bar (the fast march forms each sample's density from a per-cell cubic or by
+nested lerps instead of the exact 8-weight filter, so values differ from the
+transcription by rounding): intensity and photon_prob within 1e-5 relative of
+the transcription; every area density within 1e-5 of the pixel's total area
+density plus 1e-6 g/cm^2 (an area density off by that much moves the images by
+less than their bar; the largest difference relative to the total is printed);
+bit-identical from run to run; batched images bit-identical to one view per
+launch (the same march in every kernel).
 Scenes: the random volumes, poses, detectors and steps and the 1..16-material
 scenes of the exact tests, air outside on and off, and strided rows of the
 clinical 512x512x400 CT through the 1536^2 AP view and batch poses.
@@ -23,6 +25,7 @@ from tests.test_gpu_forward import _materials_scene, _random_scene
 pytestmark = pytest.mark.gpu
 
 REL = 1e-5
+AD_ATOL = 1e-6  # g/cm^2: the absolute floor of the area-density bar
 
 
 def _fast(case):
@@ -42,7 +45,7 @@ def _check_close(name, gpu_out, ora_out):
     diff = np.abs(AD.astype(np.float64) - ADo).max(0)
     rel = np.where(tot > 0, diff / np.where(tot > 0, tot, 1.0), diff)
     worst = float(rel.max()) if rel.size else 0.0
-    assert worst <= REL, f"{name}: area density off by {worst:.3g} of the pixel total"
+    assert np.all(diff <= REL * tot + AD_ATOL), f"{name}: area density off by {worst:.3g} of the pixel total"
     return worst
 
 
