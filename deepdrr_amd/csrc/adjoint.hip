@@ -14,10 +14,13 @@
 // handed over when it leaves, and accumulates the pose gradient from
 //   dL/dp_t = wt_t [ (sum_d w_d h_d) grad(rho) + rho grad(sum_d h_d w_d) ]
 // (the second term vanishes in a cell whose labels agree).  Nothing is stored
-// per sample.  drr_adjoint scatters with global float atomics; drr_adjoint_jobs
-// runs many views in one persistent launch whose workgroups gather the
-// scatter of their 16x16-ray tile in LDS, slab by slab along the beam, and
-// add it to the gradient volume once per touched voxel.
+// per sample; in a cell whose labels agree, pass 2 forms the corner sums over
+// the cell's samples at once, in closed form (AdjRay::cell_sums, equal to the
+// per-sample sums up to rounding).  drr_adjoint scatters with
+// global float atomics; drr_adjoint_jobs runs many views in one persistent
+// launch whose workgroups gather the scatter of their 16x16-ray tile in LDS,
+// slab by slab along the beam, and add it to the gradient volume once per
+// touched voxel.
 #include <hip/hip_runtime.h>
 
 #include "drr_internal.h"
