@@ -48,12 +48,16 @@ def _ray_setup(view, step, pixels=None):
     mrl = F32(view.max_ray_length)
     mn = np.zeros(len(u), F32)
     mx = np.full(len(u), mrl if mrl > 0 else np.inf, F32)
-    for a in range(3):
-        reci = (F32(1) / ri[:, a]).astype(F32)
-        a0 = ((lo[a] - s[a]) * reci).astype(F32)
-        a1 = ((hi[a] - s[a]) * reci).astype(F32)
-        mn = np.fmax(mn, np.fmin(a0, a1))
-        mx = np.fmin(mx, np.fmax(a0, a1))
+    # a zero direction component divides to +-inf, which leaves that axis
+    # unconstrained when the source lies between its planes and empties the
+    # ray otherwise, as the reference's skip of such components does
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for a in range(3):
+            reci = (F32(1) / ri[:, a]).astype(F32)
+            a0 = ((lo[a] - s[a]) * reci).astype(F32)
+            a1 = ((hi[a] - s[a]) * reci).astype(F32)
+            mn = np.fmax(mn, np.fmin(a0, a1))
+            mx = np.fmin(mx, np.fmax(a0, a1))
     minA = np.fmin(L, mn)
     maxA = np.fmax(F32(0), mx)
     N = np.ceil(((maxA - minA) / F32(step)).astype(F32)).astype(np.int64)

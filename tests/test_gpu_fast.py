@@ -17,6 +17,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from deepdrr_amd import phantoms
 from deepdrr_amd._abi_types import MARCH_FAST
 from tests import scenes
 from tests.scenes import GpuRunner
@@ -147,6 +148,24 @@ def test_fast_clinical_rows(gpu):
         plan.close()
         for i, o in enumerate(outs):
             assert np.array_equal(o[0].view(np.uint32), I_b[i].view(np.uint32)), f"view {i}"
+    finally:
+        r.close()
+
+
+@pytest.mark.parametrize("det,step", [((11, 11), 0.1), ((9, 7), 0.1), ((11, 11), 1.3)])
+def test_fast_axis_aligned_rays(gpu, det, step):
+    """AP views on odd detectors: the central pixel's ray runs exactly along a
+    volume axis (two direction components exactly zero for 11x11, one for
+    9x7), and the central row and column have one zero component each, so the
+    exit distance along those axes is infinite."""
+    vol = phantoms.small_ct(24, 0)
+    case = _fast(scenes.make_case(f"aligned{det[0]}x{det[1]}_step{step}", [vol],
+                                  phantoms.ap_device(det[0], det[1], 4.0).get_camera_projection(), step=step))
+    print(f"case {case.name} (fast)", flush=True)
+    r = GpuRunner(case)
+    try:
+        w = _check_close(case.name, r.project(), _oracle(case))
+        print(f"{case.name}: largest area-density difference {w:.3g} of the pixel total")
     finally:
         r.close()
 
