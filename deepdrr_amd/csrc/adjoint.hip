@@ -178,13 +178,14 @@ struct AdjRay {
     bool live;
     float graw[M];
     float acc[8];   // sum over the cell's samples of G w_d: density gradient of corner d
-    float acc1[8];  // sum of alpha G w_d (pose only)
+    float acc1[8];  // sum of (alpha - aref) G w_d (pose only)
+    float aref;     // the ray's volume entry: alpha-weighted sums are taken about it
     // (pose kernel) density sums carried into the open cell through the face
     // it was entered by, held apart from acc so the pose term sees the cell's
     // own sums
     float carry[8];
     float hu;       // dL/d(ad_raw) of the material of a cell whose labels agree
-    // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t alpha_t dL/dp_t, in FP64
+    // Pose sums S0 = sum_t dL/dp_t, S1 = sum_t (alpha_t - aref) dL/dp_t, in FP64
     // (thousands of terms of both signs), in registers.  Within a cell, dL/dp_t =
     // G_t grad(rho)(p_t) [+ wt rho_t grad(h)(p_t) if the labels differ, h_d =
     // dL/d(ad_raw) of corner d's material], and grad(rho) along x is
@@ -225,6 +226,7 @@ struct AdjRay {
 #pragma unroll
             for (int k = 0; k < 6; k++) S[k] = 0.0;
         march_setup_v1(g, udx, vdx, step, r);
+        aref = r.mnv;
         live = r.trace;
         alpha = r.minA;
         t = 0;
@@ -334,7 +336,7 @@ struct AdjRay {
     // C4 = sum v^4 = h^4 n (n^2 - 1) (3 n^2 - 7) / 240:
     //   sum_k w_d = n xyz + C2 (x'y'z + x'yz' + xy'z'),
     //   sum_k alpha_k w_d = alpha_m sum_k w_d + C2 (x'yz + xy'z + xyz') + C4 x'y'z',
-    // with alpha_m the midpoint's alpha.  Both are added times hu (the
+    // with alpha_m the midpoint's alpha (taken about aref, as acc1 is).  Both are added times hu (the
     // material's dL/d(ad_raw)), as the per-sample loop adds G w_d with G = hu
     // times the weights' sum, which is 1 to rounding.
     __device__ __forceinline__ void cell_sums(int n, float h, float am, float fx, float fy, float fz, float rx,
@@ -347,7 +349,7 @@ struct AdjRay {
         const float h2 = h * h, q = fmaf(nf, nf, -1.0f);
         const float C2 = h2 * nf * q * (1.0f / 12.0f);
         const float C4 = h2 * h2 * nf * q * fmaf(3.0f * nf, nf, -7.0f) * (1.0f / 240.0f);
-        const float amid = am + um;
+        const float amid = (am - aref) + um;
         // per (b, c) pair: yz, y'z + yz', y'z'; then each corner (a, b, c)
 #pragma unroll
         for (int bc = 0; bc < 4; bc++) {
@@ -436,9 +438,12 @@ struct AdjRay {
             // the loop below when the ray ends in the cell.
             bool cf = false;
             if (act && c.uni && t >= 1) {
-                const float nex = fminf(fmaf(-ex, jx, hx), fminf(fmaf(-ey, jy, hy), fmaf(-ez, jz, hz)));
-                const int n = min(max((int)ceilf(nex), 1), N - 1 - t);
                 const float a1 = alpha + step, s1 = a1 - alpha, s2 = (a1 + step) - a1;
+                // the samples before the exit face at the actual increment s1 (a
+                // far source moves alpha by up to a quarter step more, or not at all)
+                const float nex = fminf(fmaf(-ex, jx, hx), fminf(fmaf(-ey, jy, hy), fmaf(-ez, jz, hz))) *
+                                  (s1 > 0.0f ? step / s1 : INFINITY);
+                const int n = min(max((int)ceilf(nex), 1), N - 1 - t);
                 const float an = fmaf((float)n, s1, alpha);  // alpha after the n samples
                 if (n >= 1 && s1 == s2 && (__float_as_uint(an) ^ __float_as_uint(alpha)) < 0x00800000u &&
                     an - s1 <= r.mxv) {
@@ -477,7 +482,7 @@ struct AdjRay {
 #pragma unroll
                         for (int k = 0; k < 3; k++) {
                             add_S(k, (double)(wr * gh[k]));
-                            add_S(k + 3, (double)(alpha * wr * gh[k]));
+                            add_S(k + 3, (double)((alpha - aref) * wr * gh[k]));
                         }
                     }
                 }
@@ -485,7 +490,7 @@ struct AdjRay {
 #pragma unroll
                 for (int d = 0; d < 8; d++) acc[d] = fmaf(G, w[d], acc[d]);
                 if (POSE) {
-                    const float aG = alpha * G;
+                    const float aG = (alpha - aref) * G;
 #pragma unroll
                     for (int d = 0; d < 8; d++) acc1[d] = fmaf(aG, w[d], acc1[d]);
                 }
@@ -510,7 +515,9 @@ struct AdjRay {
         if (!r.trace) return;
         // p = A x + b - 1 with x = s_w + alpha r_w; perturbed x' = R(w)^T (x - c - t) + c:
         // dp/dt = -A, dp/dw = A [x - c]_x  ->  dL/dt = -A^T S0,
-        // dL/dw = (A^T S0) x (s_w - c) + (A^T S1) x r_w
+        // dL/dw = (A^T S0) x (s_w + aref r_w - c) + (A^T S1) x r_w with S1 summed
+        // over alpha - aref (a far source's alpha is thousands of times the
+        // volume's size; the two terms would cancel to a few digits in FP32)
         const float *A = g.A;
         const double S0[3] = {S[0], S[1], S[2]}, S1[3] = {S[3], S[4], S[5]};
         double V0[3], V1[3];
@@ -519,9 +526,10 @@ struct AdjRay {
             V0[j] = (double)A[j] * S0[0] + (double)A[4 + j] * S0[1] + (double)A[8 + j] * S0[2];
             V1[j] = (double)A[j] * S1[0] + (double)A[4 + j] * S1[1] + (double)A[8 + j] * S1[2];
         }
-        const double q[3] = {(double)sw_world[0] - center[0], (double)sw_world[1] - center[1],
-                             (double)sw_world[2] - center[2]};
         const double rw[3] = {r.rx, r.ry, r.rz};
+        const double q[3] = {(double)sw_world[0] - center[0] + (double)aref * rw[0],
+                             (double)sw_world[1] - center[1] + (double)aref * rw[1],
+                             (double)sw_world[2] - center[2] + (double)aref * rw[2]};
         gpose[0] -= V0[0];
         gpose[1] -= V0[1];
         gpose[2] -= V0[2];

@@ -439,17 +439,18 @@ __device__ __forceinline__ void fast_exact(const Cell &c, float (&ad)[M], bool e
 template <int M>
 __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &fc, float (&ad)[M], Accum &a,
                                          float sx, float sy, float sz, float rix, float riy, float riz,
-                                         float step, float &alpha, int &t, int tlim) {
+                                         float step, float &alpha, int &t, int tlim, float amax) {
     // Cell by cell: every lane still marching enters the cell of its current
     // sample together (one execution of the long cell-entry block for the whole
     // wave), then the wave runs the samples of those cells until every lane has
     // taken its own.  Lanes whose cells end early idle for the rest of that inner
     // loop, which costs less than running the entry block once for each group
     // of lanes that leave together.  A cell's sample count n is fixed on entry:
-    // the samples at u = 0, step, 2 step, ... before the exit distance (at least
-    // the entry sample, at most the samples left before tlim); a sample within
-    // the FP32 alpha sequence's drift of the face may so fall to either cell,
-    // where the density is continuous.  The inner loop's test is a wave vote and
+    // the samples at u = 0, step, 2 step, ... (for a far source, at the actual
+    // FP32 increment) before the exit distance (at least the entry sample, at
+    // most the samples left before tlim); a sample within the FP32 alpha
+    // sequence's drift of the face may so fall to either cell, where the
+    // density is continuous.  The inner loop's test is a wave vote and
     // its per-lane choices are 0/1 factors: per-lane branches and loop exits cost
     // scalar instructions (exec-mask bookkeeping), enough to become the limit in
     // a loop this short.  With g = 1, fma(g, rho, cur) = cur + rho and
@@ -484,6 +485,13 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     float hy = riy == 0.0f ? INFINITY : (riy < 0.0f ? 0.0f : jy);
     float hz = riz == 0.0f ? INFINITY : (riz < 0.0f ? 0.0f : jz);
     asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(hx), "+v"(hy), "+v"(hz));
+    // Each FP32 `alpha += step` moves alpha by step rounded to alpha's ulp, off
+    // step by up to half an ulp.  Below 2^14 steps (about 1.6 m at 0.1 mm) that
+    // is under 2^-11 of a step and the samples a cell's count assumes stay
+    // within rounding of its face; beyond (a far source), the count uses the
+    // actual increment, so no sample is taken past the exit face (at 4 km
+    // alpha moves a quarter step more, or not at all).
+    const bool far = amax > 16384.0f * step;
     float sq = 0.0f;  // ad[q] while q (the second material of two-label cells) is >= 0
     int q = -1;
     // two labels: the q-indicators' x-differences (read only where tw, so not
@@ -547,7 +555,12 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
             // the samples before the exit face, clamped to [1, tlim - t] after
             // the conversion (which saturates infinities and takes NaN to 0), so
             // every entry takes at least one sample
-            n = min(max((int)ceilf(fc.nex), 1), tlim - t);
+            float nex = fc.nex;
+            if (far) {
+                const float s1 = (alpha + step) - alpha;  // the actual increment (0: alpha stays)
+                nex = s1 > 0.0f ? nex * (step / s1) : INFINITY;
+            }
+            n = min(max((int)ceilf(nex), 1), tlim - t);
         }
         if (__ballot(tw) != 0)
             fast_cell<true>(c, fc, a, sq, e0, e1, e2, e3, tw, n, sx, sy, sz, rix, riy, riz, step, alpha);
@@ -699,7 +712,7 @@ __device__ __forceinline__ void march_v1(const Geo1 &g, const VolDev &vol, int u
                     ++t;
                 }
                 FastCell fc;  // no cell entered: the first sample enters one
-                fast_run<M>(vol, c, fc, ad, acc3, sx, sy, sz, rix, riy, riz, step, alpha, t, tlim);
+                fast_run<M>(vol, c, fc, ad, acc3, sx, sy, sz, rix, riy, riz, step, alpha, t, tlim, r.mxv);
             } else {
                 while (t < tlim) {
                     sample_v1<M>(vol, c, ad, acc3, sample_pos(sx, alpha, rix), sample_pos(sy, alpha, riy),

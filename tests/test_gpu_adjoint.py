@@ -163,6 +163,38 @@ def test_adjoint_axis_aligned_rays(gpu):
         dp.close()
 
 
+def test_adjoint_far_source(gpu):
+    """A source 2 km from the volume: each FP32 `alpha += step` moves alpha by
+    0.125 mm at this distance, not 0.1, so a cell's sample count must use the
+    actual increment.  Density and pose gradients against the dense FP64
+    autograd gradients (the dense model samples the same FP32 alpha sequence)."""
+    dens, labels, W, _, step = _scene()
+    projs = [phantoms.pose_projection(12, 12, 4.0, p, d, f, sdd=4.0e6)
+             for p, d, f in phantoms.batch_poses(2, seed=5, radius=2.0, cone_deg=40)]
+    dp = DifferentiableProjector(labels, MATS, W, step=step, spectrum=synthetic_120kv())
+    try:
+        gI = np.random.default_rng(3).standard_normal((2, 12, 12)).astype(np.float32)
+        d_t = torch.tensor(dens, device=gpu, requires_grad=True)
+        th_t = torch.zeros(6, device=gpu, requires_grad=True)
+        print("adjoint, far source: forward + backward on GPU", flush=True)
+        I = dp(d_t, th_t, projs)
+        (I * torch.tensor(gI, device=gpu)).sum().backward()
+        gD = d_t.grad.detach().cpu().numpy().astype(np.float64)
+        gth = th_t.grad.detach().cpu().numpy().astype(np.float64)
+        ref = _dense(dp, projs)
+        d64 = torch.tensor(dens.astype(np.float64), requires_grad=True)
+        th64 = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+        I64 = ref.render(d64, _onehot(labels), th64)
+        np.testing.assert_allclose(I.detach().cpu().numpy(), I64.detach().numpy(), rtol=2e-5)
+        (I64 * torch.tensor(gI, dtype=torch.float64)).sum().backward()
+        rel = np.linalg.norm(gD - d64.grad.numpy()) / np.linalg.norm(d64.grad.numpy())
+        err = np.linalg.norm(gth - th64.grad.numpy()) / np.linalg.norm(th64.grad.numpy())
+        print(f"far source: density grad rel L2 {rel:.2e}, pose grad rel L2 {err:.2e}")
+        assert rel < 1e-4 and err < 1e-4
+    finally:
+        dp.close()
+
+
 def test_adjoint_density_only_and_no_pose(gpu):
     """theta without grad: only the density gradient is produced and it matches."""
     dens, labels, W, projs, step = _scene(n=16, n_views=2, det=10, seed=3)

@@ -170,6 +170,33 @@ def test_fast_axis_aligned_rays(gpu, det, step):
         r.close()
 
 
+@pytest.mark.parametrize("where", ["far2e4", "far4e6", "inside"])
+def test_fast_far_and_inside_sources(gpu, where):
+    """A far source (alpha near 2e4 or 4e6 mm, where each FP32 `alpha += step`
+    rounds by up to a quarter step or more: the fast march places its samples by
+    the actual alpha sequence, so they bunch as the exact march's do) and a
+    source inside the volume (alpha from 0, where the increments change from
+    binade to binade)."""
+    vol = phantoms.small_ct(32, seed=3)
+    if where == "inside":
+        d = np.array([0.3, 1.0, 0.2])
+        proj = phantoms.pose_projection(16, 12, 2.0, [1.0, -2.0, 0.5], d / np.linalg.norm(d), 0.5, sdd=20.0)
+        case = scenes.make_case("inside_source", [vol], proj)
+    else:
+        sdd = 2.0e4 if where == "far2e4" else 4.0e6
+        case = scenes.make_case(f"far_source_{sdd:g}", [vol],
+                                phantoms.ap_device(16, 12, 4.0, sdd=sdd).get_camera_projection(),
+                                max_ray_length=4.0 * sdd)
+    _fast(case)
+    print(f"case {case.name} (fast)", flush=True)
+    r = GpuRunner(case)
+    try:
+        w = _check_close(case.name, r.project(), _oracle(case))
+        print(f"{case.name}: largest area-density difference {w:.3g} of the pixel total")
+    finally:
+        r.close()
+
+
 def test_fast_scene_flag_checked(gpu):
     """march must be DRR_MARCH_EXACT or DRR_MARCH_FAST."""
     from deepdrr_amd import _abi
