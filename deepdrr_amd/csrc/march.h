@@ -446,8 +446,8 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     // taken its own.  Lanes whose cells end early idle for the rest of that inner
     // loop, which costs less than running the entry block once for each group
     // of lanes that leave together.  A cell's sample count n is fixed on entry:
-    // the samples at u = 0, step, 2 step, ... (for a far source, at the actual
-    // FP32 increment) before the exit distance (at least the entry sample, at
+    // the samples at u = 0, step, 2 step, ... (for a far source, at the FP32
+    // increment) before the exit distance (at least the entry sample, at
     // most the samples left before tlim); a sample within the FP32 alpha
     // sequence's drift of the face may so fall to either cell, where the
     // density is continuous.  The inner loop's test is a wave vote and
@@ -484,14 +484,27 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
     float hx = rix == 0.0f ? INFINITY : (rix < 0.0f ? 0.0f : jx);
     float hy = riy == 0.0f ? INFINITY : (riy < 0.0f ? 0.0f : jy);
     float hz = riz == 0.0f ? INFINITY : (riz < 0.0f ? 0.0f : jz);
-    asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(hx), "+v"(hy), "+v"(hz));
     // Each FP32 `alpha += step` moves alpha by step rounded to alpha's ulp, off
     // step by up to half an ulp.  Below 2^14 steps (about 1.6 m at 0.1 mm) that
     // is under 2^-11 of a step and the samples a cell's count assumes stay
-    // within rounding of its face; beyond (a far source), the count uses the
-    // actual increment, so no sample is taken past the exit face (at 4 km
-    // alpha moves a quarter step more, or not at all).
-    const bool far = amax > 16384.0f * step;
+    // within rounding of its face.  Beyond (a far source: at 4 km alpha moves a
+    // quarter step more, or not at all) the exit distances are scaled by step
+    // over the larger of the increments at the ray's current alpha and at its
+    // largest, those of the two binades a far ray spans, so a cell's count ends
+    // at its exit face (a ray that also crosses lower binades, starting near its
+    // source, has their increments within 3/4 of an ulp of amax of these); where
+    // alpha stops moving, every entry takes one sample.
+    if (amax > 16384.0f * step) {
+        const float s_lo = (alpha + step) - alpha, s_hi = (amax + step) - amax;
+        const float fac = (s_lo > 0.0f && s_hi > 0.0f) ? step / fmaxf(s_lo, s_hi) : 0.0f;
+        jx *= fac;
+        jy *= fac;
+        jz *= fac;
+        hx = fac > 0.0f ? hx * fac : 0.0f;
+        hy = fac > 0.0f ? hy * fac : 0.0f;
+        hz = fac > 0.0f ? hz * fac : 0.0f;
+    }
+    asm volatile("" : "+v"(jx), "+v"(jy), "+v"(jz), "+v"(hx), "+v"(hy), "+v"(hz));
     float sq = 0.0f;  // ad[q] while q (the second material of two-label cells) is >= 0
     int q = -1;
     // two labels: the q-indicators' x-differences (read only where tw, so not
@@ -555,12 +568,7 @@ __device__ __forceinline__ void fast_run(const VolDev &vol, Cell &c, FastCell &f
             // the samples before the exit face, clamped to [1, tlim - t] after
             // the conversion (which saturates infinities and takes NaN to 0), so
             // every entry takes at least one sample
-            float nex = fc.nex;
-            if (far) {
-                const float s1 = (alpha + step) - alpha;  // the actual increment (0: alpha stays)
-                nex = s1 > 0.0f ? nex * (step / s1) : INFINITY;
-            }
-            n = min(max((int)ceilf(nex), 1), tlim - t);
+            n = min(max((int)ceilf(fc.nex), 1), tlim - t);
         }
         if (__ballot(tw) != 0)
             fast_cell<true>(c, fc, a, sq, e0, e1, e2, e3, tw, n, sx, sy, sz, rix, riy, riz, step, alpha);
