@@ -176,7 +176,12 @@ def test_fast_far_and_inside_sources(gpu, where):
     rounds by up to a quarter step or more: the fast march places its samples by
     the actual alpha sequence, so they bunch as the exact march's do) and a
     source inside the volume (alpha from 0, where the increments change from
-    binade to binade)."""
+    binade to binade); the batched launch of the view (both tile shapes)
+    bit-identical to the single view."""
+    import torch
+
+    from deepdrr_amd.batch import BatchPlan
+
     vol = phantoms.small_ct(32, seed=3)
     if where == "inside":
         d = np.array([0.3, 1.0, 0.2])
@@ -191,8 +196,17 @@ def test_fast_far_and_inside_sources(gpu, where):
     print(f"case {case.name} (fast)", flush=True)
     r = GpuRunner(case)
     try:
-        w = _check_close(case.name, r.project(), _oracle(case))
+        out = r.project()
+        w = _check_close(case.name, out, _oracle(case))
         print(f"{case.name}: largest area-density difference {w:.3g} of the pixel total")
+        for tuning in ((8, 8, 4, 8), (16, 4, 2, 4)):
+            plan = BatchPlan(r.ctx, [r.vols[0]], [case.view], [0], neglog=False, tuning=tuning)
+            plan.launch()
+            torch.cuda.synchronize()
+            I_b, P_b = plan.fetch("intensity"), plan.fetch("photon_prob")
+            plan.close()
+            assert np.array_equal(out[0].view(np.uint32), I_b[0].view(np.uint32)), f"{case.name}, {tuning}"
+            assert np.array_equal(out[1].view(np.uint32), P_b[0].view(np.uint32)), f"{case.name}, {tuning}"
     finally:
         r.close()
 
