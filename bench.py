@@ -300,7 +300,7 @@ def measure_mc(torch, _abi, ctx, handle, geom, n_hist):
                 flights_per_history=steps / n_hist, flight_steps_per_s=steps / dt,
                 brick_steps=bsteps, brick_steps_per_history=bsteps / n_hist,
                 gather_GB_per_s=5.0 * steps / dt / 1e9,
-                majorant="local: one per 8x8x8-voxel brick",
+                majorant="local: one per 16x16x16-voxel brick",
                 scatter_fraction=float(img[1:].sum() / img.sum()), detector="256^2 at 1.164 mm")
 
 

@@ -15,7 +15,7 @@
 //
 // Local majorants: one global majorant (the densest voxel's attenuation, here
 // a titanium rod) made a photon in soft tissue take ~155 flights, most of them
-// virtual.  The volume is cut into 8x8x8-voxel bricks, and each brick stores
+// virtual.  The volume is cut into 16x16x16-voxel bricks, and each brick stores
 // the component-wise maxima of density x (photoelectric, Compton, Rayleigh)
 // coefficient over its voxels (k_mc_bricks, per call); its majorant at energy
 // E is maj_pe f_pe(E) + maj_c f_c(E) + maj_r f_r(E), which bounds every
@@ -142,7 +142,7 @@ __device__ __forceinline__ void cross_sections(const drr_mc_params_t &p, float E
     fr = exp2f(-p.k_rayleigh * l);
 }
 constexpr float E_MIN = 5.0f;  // keV: lower energies are absorbed locally
-constexpr int BRICK = 8;       // voxels per brick edge
+constexpr int BRICK = 16;      // voxels per brick edge (a power of two)
 
 struct Bricks {
     const float4 *maj;  // [nbz][nby][nbx]: (max rho a_pe, max rho a_c, max rho a_r, 0)
@@ -151,20 +151,18 @@ struct Bricks {
 
 }  // namespace
 
-// One wave per brick: lane l reads voxel row (l & 7, l >> 3) of each of the
-// brick's 8 z-slices (8 contiguous records per row), then the wave reduces.
+// One wave per brick: lane l reads voxels l, l + 64, ... of the brick in x-fastest
+// order (rows of BRICK contiguous records), then the wave reduces.
 __global__ __launch_bounds__(256) void k_mc_bricks(const uint2 *__restrict__ vox, int nx, int ny, int nz,
                                                    drr_mc_params_t p, Bricks b, int64_t n_bricks) {
     const int64_t bi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (bi >= n_bricks) return;  // whole waves leave together
     const int bx = (int)(bi % b.nb[0]), by = (int)((bi / b.nb[0]) % b.nb[1]), bz = (int)(bi / ((int64_t)b.nb[0] * b.nb[1]));
-    const int i = bx * BRICK + (lane & 7), j = by * BRICK + (lane >> 3);
     float mpe = 0.0f, mc = 0.0f, mr = 0.0f;
-    if (i < nx && j < ny) {
-        for (int dz = 0; dz < BRICK; dz++) {
-            const int k = bz * BRICK + dz;
-            if (k >= nz) break;
+    for (int v = lane; v < BRICK * BRICK * BRICK; v += 64) {
+        const int i = bx * BRICK + v % BRICK, j = by * BRICK + (v / BRICK) % BRICK, k = bz * BRICK + v / (BRICK * BRICK);
+        if (i < nx && j < ny && k < nz) {
             const uint2 rv = vox[((int64_t)k * ny + j) * nx + i];
             const float rho = __uint_as_float(rv.x);
             const int m = (int)rv.y;
@@ -263,7 +261,7 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
             for (int a = 0; a < 3; a++) {
                 const float qp = A[4 * a] * ph.x[0] + A[4 * a + 1] * ph.x[1] + A[4 * a + 2] * ph.x[2] + A[4 * a + 3] + 0.5f;
                 const float dq = A[4 * a] * ph.d[0] + A[4 * a + 1] * ph.d[1] + A[4 * a + 2] * ph.d[2];
-                bi[a] = min(max((int)floorf(qp * 0.125f), 0), bk.nb[a] - 1);
+                bi[a] = min(max((int)floorf(qp * (1.0f / BRICK)), 0), bk.nb[a] - 1);
                 if (dq > 0.0f) {
                     tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;
                     td[a] = (float)BRICK / dq;

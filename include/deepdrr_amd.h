@@ -326,7 +326,7 @@ typedef struct {
     /* mu_x[m] (1/mm per g/cm^3) = a_x[m] * (E / 60 keV)^(-k_x) */
     float a_pe[DRR_MAX_MATERIALS], a_compton[DRR_MAX_MATERIALS], a_rayleigh[DRR_MAX_MATERIALS];
     float k_pe, k_compton, k_rayleigh;
-    float max_density;                /* unused (the majorants are per 8^3-voxel brick) */
+    float max_density;                /* unused (the majorants are per 16^3-voxel brick) */
     float source_world[3];            /* point source */
     float world_from_ijk[12];         /* 3x4 row-major */
     float ijk_from_world[12];         /* 3x4 row-major */
@@ -351,7 +351,7 @@ typedef struct {
  * of Woodcock flights (each ends in a tentative collision or leaves the
  * volume) and n_brick_steps (optional, device int64) the number of brick
  * majorants they crossed.  All four accumulate.  Delta tracking uses a local
- * majorant per 8x8x8-voxel brick (built per call in stream-ordered memory).
+ * majorant per 16x16x16-voxel brick (built per call in stream-ordered memory).
  * The volume is sampled at voxel centres (nearest voxel), IJK box
  * [-0.5, n-0.5]. */
 int drr_mc_scatter(drr_ctx *ctx, const drr_volume *vol, const drr_mc_params_t *params,

@@ -6,7 +6,7 @@
  * (projector.py:527-531); this restates the MC-GPU method the engine
  * implements: point source, spectrum sampling by the cumulative pdf, emission
  * uniform in a cone, Woodcock tracking against a local majorant per
- * 8x8x8-voxel brick (component-wise maxima of density x the three process
+ * 16x16x16-voxel brick (component-wise maxima of density x the three process
  * coefficients; a flight spends its optical depth brick by brick), photoelectric /
  * Compton (Klein-Nishina, Kahn's method) / Rayleigh (Thomson) interactions,
  * scoring on the detector as [primary, single Compton, single Rayleigh,
@@ -115,7 +115,7 @@ static void cross_sections(const drr_mc_params_t *p, float E, float *fpe, float 
     *fr = powf(E / 60.0f, -p->k_rayleigh);
 }
 
-#define BRICK 8
+#define BRICK 16
 
 typedef struct {
     int nb[3];
@@ -151,7 +151,7 @@ static float flight(const bricks_t *b, const float *A, const float *x, const flo
     for (int a = 0; a < 3; a++) {
         float qp = A[4 * a] * x[0] + A[4 * a + 1] * x[1] + A[4 * a + 2] * x[2] + A[4 * a + 3] + 0.5f;
         float dq = A[4 * a] * d[0] + A[4 * a + 1] * d[1] + A[4 * a + 2] * d[2];
-        int c = (int)floorf(qp * 0.125f);
+        int c = (int)floorf(qp * (1.0f / BRICK));
         bi[a] = c < 0 ? 0 : (c > b->nb[a] - 1 ? b->nb[a] - 1 : c);
         if (dq > 0.0f) {
             tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;

@@ -47,18 +47,21 @@ def test_thomson_sampler():
     _hist_check(c, lambda x: 1.0 + x * x)
 
 
-def _slab_setup(E_keV=60.0, det=16, n=20, rho=1.0):
+def _slab_setup(E_keV=60.0, det=16, n=20, rho=1.0, pixel=1.0):
     vol = phantoms.uniform_volume((n, n, n), spacing=(1, 1, 1), density=rho, material="soft tissue")
-    dev = phantoms.ap_device(det, det, 1.0, sdd=400.0)
+    dev = phantoms.ap_device(det, det, pixel, sdd=400.0)
     proj = dev.get_camera_projection()
     spectrum = np.array([[E_keV * 1000.0, 1.0]])
     return vol, proj, spectrum
 
 
-def _expected_primary(p, n_hist, E, boxes):
+def _expected_primary(p, n_hist, E, boxes, ss=1):
     """E * N * (Omega_pix / Omega_cone) * exp(-sum_b mu_b L_b): boxes are
     (lo, hi, mu) in IJK with later boxes replacing earlier ones where they
-    overlap (each given as its own mu minus the enclosing box's)."""
+    overlap (each given as its own mu minus the enclosing box's).  The
+    transmission is averaged over ss x ss rays spread evenly over the pixel
+    (ss = 1: the pixel's centre ray), which matters where a box's edge falls
+    inside a pixel."""
     src = np.array(p.source_world[:])
     o = np.array(p.detector_origin[:])
     du = np.array(p.detector_u[:])
@@ -67,19 +70,21 @@ def _expected_primary(p, n_hist, E, boxes):
     H, W = p.det_height, p.det_width
     expect = np.zeros((H, W))
     frac = mc.pixel_solid_angles(p) / mc.cone_solid_angle(p)
+    sub = (np.arange(ss) + 0.5) / ss
+    q0 = Wi[:, :3] @ src + Wi[:, 3]
     for iv in range(H):
         for iu in range(W):
-            tgt = o + (iu + 0.5) * du + (iv + 0.5) * dv
-            d = (tgt - src) / np.linalg.norm(tgt - src)
-            q0 = Wi[:, :3] @ src + Wi[:, 3]
-            dq = Wi[:, :3] @ d
-            tau = 0.0
+            tgt = o + (iu + sub[None, :, None]) * du + (iv + sub[:, None, None]) * dv
+            d = (tgt - src).reshape(-1, 3)
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            dq = d @ Wi[:, :3].T
+            tau = np.zeros(len(d))
             for lo, hi, mu in boxes:
-                with np.errstate(divide="ignore"):
+                with np.errstate(divide="ignore", invalid="ignore"):
                     t0 = (lo - q0) / dq
                     t1 = (hi - q0) / dq
-                tau += mu * max(0.0, np.min(np.maximum(t0, t1)) - np.max(np.minimum(t0, t1)))
-            expect[iv, iu] = E * n_hist * frac[iv, iu] * math.exp(-tau)
+                tau += mu * np.maximum(0.0, np.min(np.maximum(t0, t1), axis=1) - np.max(np.minimum(t0, t1), axis=1))
+            expect[iv, iu] = E * n_hist * frac[iv, iu] * np.mean(np.exp(-tau))
     return expect
 
 
@@ -102,32 +107,34 @@ def test_woodcock_primary_matches_beer_lambert():
 
 
 def test_local_majorants_with_a_dense_insert():
-    """A 4^3-voxel titanium insert (4.5 g/cm^3) in a soft-tissue cube: the
-    brick majorants keep the primary unbiased across bricks of very different
-    majorants (Beer-Lambert through both materials), and the flights grow far
-    less than the titanium/tissue attenuation ratio a global majorant would
-    multiply them by."""
-    vol, proj, spectrum = _slab_setup()
+    """A 4^3-voxel titanium insert (4.5 g/cm^3) at the centre of a 48^3
+    soft-tissue cube, seen over 32 mm at the centre: the brick majorants keep
+    the primary unbiased across bricks of very different majorants
+    (Beer-Lambert through both materials), and the flights grow far less than
+    the titanium/tissue attenuation ratio a global majorant would multiply them
+    by (the insert's 16^3 brick is a third of the path of a quarter of the
+    rays)."""
+    vol, proj, spectrum = _slab_setup(n=48, pixel=4.0)
     d_kji = np.ascontiguousarray(np.moveaxis(vol.data, [0, 1, 2], [2, 1, 0])).copy()
     l_kji = np.zeros_like(d_kji, dtype=np.uint8)
     mats = ["soft tissue", "titanium"]
     p = mc.make_params(mats, spectrum, proj, 400.0, vol.world_from_IJK.data)
     n_hist = 400000
     _, _, steps_uniform = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8)
-    d_kji[8:12, 8:12, 8:12] = 4.5
-    l_kji[8:12, 8:12, 8:12] = 1
+    d_kji[22:26, 22:26, 22:26] = 4.5
+    l_kji[22:26, 22:26, 22:26] = 1
     bsteps = []
     img, img2, steps = ctrans.mc(d_kji, l_kji, p, n_hist, seed=11, n_threads=8, brick_steps=bsteps)
     mu_st = sum(mc.SYNTHETIC_MU60["soft tissue"]) * 0.1
     mu_ti = sum(mc.SYNTHETIC_MU60["titanium"]) * 0.1 * 4.5
     assert mu_ti > 10 * mu_st
-    expect = _expected_primary(p, n_hist, 60.0, [(-0.5, 19.5, mu_st), (7.5, 11.5, mu_ti - mu_st)])
+    expect = _expected_primary(p, n_hist, 60.0, [(-0.5, 47.5, mu_st), (21.5, 25.5, mu_ti - mu_st)], ss=8)
     z = _z(img[0], expect, 60.0)
     assert np.mean(np.abs(z) < 4) > 0.98, z
     assert abs(np.mean(z)) < 0.5
     # the insert's rays really are darker
     assert expect.min() < 0.5 * expect.max()
-    # the insert's 8^3 brick carries the titanium majorant; a global majorant
+    # the insert's 16^3 brick carries the titanium majorant; a global majorant
     # would scale every flight count by mu_ti / mu_st
     assert steps < 0.2 * steps_uniform * (mu_ti / mu_st), (steps, steps_uniform, mu_ti / mu_st)
     assert bsteps[0] > steps
