@@ -262,13 +262,14 @@ __global__ __launch_bounds__(256) void k_mc(const uint2 *__restrict__ vox, int n
                 const float qp = A[4 * a] * ph.x[0] + A[4 * a + 1] * ph.x[1] + A[4 * a + 2] * ph.x[2] + A[4 * a + 3] + 0.5f;
                 const float dq = A[4 * a] * ph.d[0] + A[4 * a + 1] * ph.d[1] + A[4 * a + 2] * ph.d[2];
                 bi[a] = min(max((int)floorf(qp * (1.0f / BRICK)), 0), bk.nb[a] - 1);
+                const float idq = 1.0f / dq;  // one division per axis, not two
                 if (dq > 0.0f) {
-                    tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;
-                    td[a] = (float)BRICK / dq;
+                    tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) * idq;
+                    td[a] = (float)BRICK * idq;
                     sg[a] = 1;
                 } else if (dq < 0.0f) {
-                    tn[a] = ((float)(BRICK * bi[a]) - qp) / dq;
-                    td[a] = -(float)BRICK / dq;
+                    tn[a] = ((float)(BRICK * bi[a]) - qp) * idq;
+                    td[a] = -(float)BRICK * idq;
                     sg[a] = -1;
                 } else {
                     tn[a] = INFINITY;

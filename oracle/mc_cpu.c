@@ -153,13 +153,14 @@ static float flight(const bricks_t *b, const float *A, const float *x, const flo
         float dq = A[4 * a] * d[0] + A[4 * a + 1] * d[1] + A[4 * a + 2] * d[2];
         int c = (int)floorf(qp * (1.0f / BRICK));
         bi[a] = c < 0 ? 0 : (c > b->nb[a] - 1 ? b->nb[a] - 1 : c);
+        float idq = 1.0f / dq; /* as the kernel: one division per axis */
         if (dq > 0.0f) {
-            tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) / dq;
-            td[a] = (float)BRICK / dq;
+            tn[a] = ((float)(BRICK * (bi[a] + 1)) - qp) * idq;
+            td[a] = (float)BRICK * idq;
             sg[a] = 1;
         } else if (dq < 0.0f) {
-            tn[a] = ((float)(BRICK * bi[a]) - qp) / dq;
-            td[a] = -(float)BRICK / dq;
+            tn[a] = ((float)(BRICK * bi[a]) - qp) * idq;
+            td[a] = -(float)BRICK * idq;
             sg[a] = -1;
         } else {
             tn[a] = INFINITY;
