@@ -58,9 +58,15 @@ def test_mc_matches_cpu(gpu):
         run = mc.MonteCarlo(ctx, h, p)
         run.run(n_hist, seed=12345)
         g, g2, gsteps = run.results()
-        c, c2, csteps = ctrans.mc(dens, labs, p, n_hist, seed=12345)
-        print(f"steps gpu {gsteps} cpu {csteps}; energy per class gpu {g.sum((1, 2))} cpu {c.sum((1, 2))}")
+        gbsteps = run.brick_steps()
+        cb = []
+        c, c2, csteps = ctrans.mc(dens, labs, p, n_hist, seed=12345, brick_steps=cb)
+        print(f"steps gpu {gsteps} cpu {csteps}; brick steps gpu {gbsteps} cpu {cb[0]}; "
+              f"energy per class gpu {g.sum((1, 2))} cpu {c.sum((1, 2))}")
         assert abs(gsteps - csteps) <= 1e-3 * csteps
+        # the 24^3 volume has partial bricks on every axis: both walks cross the
+        # same bricks
+        assert abs(gbsteps - cb[0]) <= 1e-3 * cb[0]
         for k, name in enumerate(mc.CLASSES):
             sig = np.sqrt(g2[k] + c2[k])
             sel = sig > 0
