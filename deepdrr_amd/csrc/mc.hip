@@ -200,6 +200,9 @@ __global__ __launch_bounds__(256, 7) void k_mc(const uint2 *__restrict__ vox, in
     int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int W = p.det_width, H = p.det_height;
     const int64_t npix = (int64_t)W * H;
+    // pixel coordinates along u and v are (hit - origin) . u / |u|^2
+    const float iuu = 1.0f / (p.detector_u[0] * p.detector_u[0] + p.detector_u[1] * p.detector_u[1] + p.detector_u[2] * p.detector_u[2]);
+    const float ivv = 1.0f / (p.detector_v[0] * p.detector_v[0] + p.detector_v[1] * p.detector_v[1] + p.detector_v[2] * p.detector_v[2]);
     const float *A = p.ijk_from_world;
     unsigned long long steps = 0, bsteps = 0;
     Photon ph;
@@ -239,7 +242,8 @@ __global__ __launch_bounds__(256, 7) void k_mc(const uint2 *__restrict__ vox, in
                 const float q0 = A[4 * a] * ph.x[0] + A[4 * a + 1] * ph.x[1] + A[4 * a + 2] * ph.x[2] + A[4 * a + 3];
                 const float dq = A[4 * a] * ph.d[0] + A[4 * a + 1] * ph.d[1] + A[4 * a + 2] * ph.d[2];
                 if (dq != 0.0f) {
-                    float t0 = (-0.5f - q0) / dq, t1 = (nn[a] - 0.5f - q0) / dq;
+                    const float idq = 1.0f / dq;
+                    const float t0 = (-0.5f - q0) * idq, t1 = (nn[a] - 0.5f - q0) * idq;
                     tmin = fmaxf(tmin, fminf(t0, t1));
                     tmax = fminf(tmax, fmaxf(t0, t1));
                 } else if (q0 < -0.5f || q0 > nn[a] - 0.5f) {
@@ -368,10 +372,8 @@ __global__ __launch_bounds__(256, 7) void k_mc(const uint2 *__restrict__ vox, in
         if (t < 0.0f) continue;
         float r[3];
         for (int a = 0; a < 3; a++) r[a] = ph.x[a] + t * ph.d[a] - p.detector_origin[a];
-        const float uu = p.detector_u[0] * p.detector_u[0] + p.detector_u[1] * p.detector_u[1] + p.detector_u[2] * p.detector_u[2];
-        const float vv = p.detector_v[0] * p.detector_v[0] + p.detector_v[1] * p.detector_v[1] + p.detector_v[2] * p.detector_v[2];
-        const float fu = (r[0] * p.detector_u[0] + r[1] * p.detector_u[1] + r[2] * p.detector_u[2]) / uu;
-        const float fv = (r[0] * p.detector_v[0] + r[1] * p.detector_v[1] + r[2] * p.detector_v[2]) / vv;
+        const float fu = (r[0] * p.detector_u[0] + r[1] * p.detector_u[1] + r[2] * p.detector_u[2]) * iuu;
+        const float fv = (r[0] * p.detector_v[0] + r[1] * p.detector_v[1] + r[2] * p.detector_v[2]) * ivv;
         if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)W && fv < (float)H)) continue;
         const int iu = min((int)fu, W - 1), iv = min((int)fv, H - 1);
         const int cls = (ph.nc + ph.nr == 0) ? 0 : (ph.nc == 1 && ph.nr == 0) ? 1 : (ph.nr == 1 && ph.nc == 0) ? 2 : 3;

@@ -230,7 +230,8 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
     for (int a = 0; a < 3; a++) {
         float q0 = qcoord(A, a, x), dq = dcoord(A, a, d);
         if (dq != 0.0f) {
-            float t0 = (-0.5f - q0) / dq, t1 = ((float)n[a] - 0.5f - q0) / dq;
+            float idq = 1.0f / dq;
+            float t0 = (-0.5f - q0) * idq, t1 = ((float)n[a] - 0.5f - q0) * idq;
             tmin = fmaxf(tmin, fminf(t0, t1));
             tmax = fminf(tmax, fmaxf(t0, t1));
         } else if (q0 < -0.5f || q0 > (float)n[a] - 0.5f) {
@@ -287,7 +288,10 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
                 float tm = INFINITY;
                 for (int a = 0; a < 3; a++) {
                     float q0 = qcoord(A, a, x), dq = dcoord(A, a, d);
-                    if (dq != 0.0f) tm = fminf(tm, fmaxf((-0.5f - q0) / dq, ((float)n[a] - 0.5f - q0) / dq));
+                    if (dq != 0.0f) {
+                        float idq = 1.0f / dq;
+                        tm = fminf(tm, fmaxf((-0.5f - q0) * idq, ((float)n[a] - 0.5f - q0) * idq));
+                    }
                 }
                 t_exit = fmaxf(tm, 0.0f);
                 cross_sections(p, E, &fpe, &fc, &fr);
@@ -307,10 +311,10 @@ static int history(const float *dens, const uint8_t *lab, const int *n, const dr
     }
     float r[3];
     for (int a = 0; a < 3; a++) r[a] = x[a] + t * d[a] - O[a];
-    float uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
-    float vv = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
-    float fu = (r[0] * U[0] + r[1] * U[1] + r[2] * U[2]) / uu;
-    float fv = (r[0] * V[0] + r[1] * V[1] + r[2] * V[2]) / vv;
+    float iuu = 1.0f / (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    float ivv = 1.0f / (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    float fu = (r[0] * U[0] + r[1] * U[1] + r[2] * U[2]) * iuu;
+    float fv = (r[0] * V[0] + r[1] * V[1] + r[2] * V[2]) * ivv;
     if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)p->det_width && fv < (float)p->det_height)) {
         if (tally) tally[T_ESCAPED] += E;
         return -1;
