@@ -213,14 +213,14 @@ __global__ __launch_bounds__(256, 7) void k_mc(const uint2 *__restrict__ vox, in
     for (;;) {
         if (!ph.alive && h >= count) break;
         {
-            // a dead lane waits for a new history until 8 lanes of the wave are
+            // a dead lane waits for a new history until 4 lanes of the wave are
             // dead or none is alive, so the long start of a history runs for
             // several lanes at once (each lane keeps its own sequence of
-            // histories, so results do not change; 8 measured 1-2 % faster
-            // than starting at once, 16 the same, 32 slower)
+            // histories, so results do not change; measured against starting
+            // at once: 4 lanes +3 %, 8 +1-2 %, 12 and 16 no change, 32 slower)
             const unsigned long long alive = __ballot(ph.alive);
             const int ndead = __popcll(__ballot(!ph.alive));
-            if (!ph.alive && alive != 0ull && ndead < 8) continue;
+            if (!ph.alive && alive != 0ull && ndead < 4) continue;
         }
         if (!ph.alive) {
             rng = seed_history(seed, (uint64_t)(first + h));
